@@ -1,0 +1,142 @@
+/*
+ * intspmv.h -- C ABI of the MI355X (gfx950) SpMV library libintspmv.so.
+ *
+ * The reference's kernels are static C functions called with raw host
+ * pointers (intSpmv_main.c:382-1566).  Each entry point below replaces one of
+ * them with a hand-written gfx950 HIP kernel that takes DEVICE pointers to the
+ * same arrays the reference builders produce (see intspmv_host.h).
+ *
+ * Conventions
+ *   - Every function returns INTSPMV_OK (0) or a negative INTSPMV_ERR_* code;
+ *     intspmv_strerror() names it.  Nothing calls exit().
+ *   - Launches are asynchronous on the context's stream (intspmv_set_stream).
+ *     One context per host thread; a plan must not be used by two launches
+ *     that can overlap in time (its split-row counters are shared).
+ *   - y semantics: all spmv_* entry points OVERWRITE y (y = A*x), including
+ *     rows with no entries.  The reference's AXT kernels instead accumulate
+ *     into a y their caller zeroed (intSpmv_main.c:1421, 1492, 1591); a host
+ *     harness that zeroes y first observes the same result.
+ *   - A plan (intspmv_*_plan) is the per-matrix analysis done once, on the
+ *     host, like the reference's getFormatData* conversion step: it cuts the
+ *     rows into wave-sized work blocks and owns the small device scratch used
+ *     to combine rows that are split across waves.
+ */
+#ifndef INTSPMV_H
+#define INTSPMV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    INTSPMV_OK = 0,
+    INTSPMV_ERR_ARG = -1,         /* invalid argument / inconsistent sizes */
+    INTSPMV_ERR_HIP = -2,         /* a HIP runtime call failed */
+    INTSPMV_ERR_NOMEM = -3,       /* device or host allocation failed */
+    INTSPMV_ERR_UNSUPPORTED = -4, /* combination not implemented */
+    INTSPMV_ERR_OVERFLOW = -5,    /* a size does not fit the 32-bit index type */
+    INTSPMV_ERR_NODEVICE = -6     /* no HIP device visible */
+};
+
+/* element types of stored values / vectors */
+enum { INTSPMV_F64 = 0, INTSPMV_F32 = 1, INTSPMV_BF16 = 2 };
+/* AXT modes (str_matAXT.mode, intSpmv_main.c:1114) */
+enum { INTSPMV_NOC = 0, INTSPMV_COM = 1 };
+
+typedef struct intspmv_ctx intspmv_ctx;
+typedef struct intspmv_plan intspmv_plan;
+
+/* ---- context, memory, errors ------------------------------------------ */
+int intspmv_create(intspmv_ctx **ctx, int device);
+int intspmv_destroy(intspmv_ctx *ctx);
+int intspmv_set_stream(intspmv_ctx *ctx, void *hip_stream); /* NULL = null stream */
+int intspmv_sync(intspmv_ctx *ctx);
+const char *intspmv_strerror(int status);
+const char *intspmv_last_hip_error(intspmv_ctx *ctx);
+int intspmv_device_count(int *count);
+int intspmv_malloc(intspmv_ctx *ctx, void **dptr, size_t bytes);
+int intspmv_free(intspmv_ctx *ctx, void *dptr);
+int intspmv_memcpy_htod(intspmv_ctx *ctx, void *dst, const void *src, size_t bytes);
+int intspmv_memcpy_dtoh(intspmv_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- CSR ----------------------------------------------------------------
+ * Replaces cf_CSR (intSpmv_main.c:382-396).
+ * spmv_csr_ordered: one lane per row, fused multiply-add in CSR order from
+ *   +0.0 -- the same bits as cf_CSR compiled with contraction.
+ * spmv_csr: nnz-balanced wave blocks; rows that fit a block (all rows of up to
+ *   `long_units` entries) get the same bits as spmv_csr_ordered, longer rows
+ *   are reduced by a wave tree and split across waves. */
+int intspmv_csr_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row_host,
+                     intspmv_plan **plan);
+int spmv_csr_ordered(intspmv_ctx *ctx, uint32_t nrows, const double *val, const uint32_t *col,
+                     const uint32_t *row, const double *x, double *y);
+int spmv_csr(intspmv_ctx *ctx, const intspmv_plan *plan, const double *val,
+             const uint32_t *col, const uint32_t *row, const double *x, double *y);
+
+/* ---- AXC ----------------------------------------------------------------
+ * Replaces int_axc (intSpmv_main.c:711-732): per brick the halving-tree sum
+ * of val*x, accumulated per row in brick order. */
+int intspmv_axc_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *brp_host,
+                     intspmv_plan **plan);
+int spmv_axc(intspmv_ctx *ctx, const intspmv_plan *plan, const double *ax, const uint32_t *brp,
+             double *y);
+
+/* ---- K1 -----------------------------------------------------------------
+ * Replaces int_k1 + orderArrayK1 (intSpmv_main.c:943-976) and int_k1p
+ * (intSpmv_main.c:1026-1066).  With permi == NULL the permuted result
+ * res[chunkNum*8] is written (int_k1); otherwise y[permi[i]] (int_k1p, which
+ * masks the padded lanes of the last chunk). */
+int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkNum, const uint32_t *nmc_host,
+                    intspmv_plan **plan);
+int spmv_k1(intspmv_ctx *ctx, const intspmv_plan *plan, const double *val, const uint32_t *col,
+            const uint32_t *chp, const uint32_t *nmc, const uint32_t *permi, const double *x,
+            double *y);
+/* orderArrayK1 alone: y[permi[i]] = res[i] for i < nrows. */
+int spmv_k1_order(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *permi, const double *res,
+                  double *y);
+
+/* ---- AXT ----------------------------------------------------------------
+ * Replaces int_axt_noc_h1 (intSpmv_main.c:1385-1404), int_axt_noc
+ * (intSpmv_main.c:1443-1474) and int_axt_com_h1 (intSpmv_main.c:1517-1566).
+ * The descriptor carries the lengths of str_matAXT plus colN, the number of
+ * occupied tile columns (NOC th>1) or slots (th==1): the reference's local
+ * totalColumns of getArraysLenAXT (intSpmv_main.c:1127-1141). */
+typedef struct {
+    uint32_t nrows, nnz;
+    uint32_t tileHW, tileH, tileN;
+    uint32_t lenAX, lenSEC, colN;
+    uint32_t bs, log; /* COM only */
+    int32_t mode;     /* INTSPMV_NOC or INTSPMV_COM */
+    int32_t val_type; /* stored value type: F64, F32 or BF16 */
+    int32_t x_type;   /* embedded / gathered x type: F64 or F32 */
+    int32_t layout;   /* INTSPMV_AXT_INTERLEAVED or INTSPMV_AXT_PLANAR */
+} intspmv_axt_desc;
+
+/* INTERLEAVED: the reference's ax: per brick-row 8 values then 8 x, both of
+ *   the same type (F64/F64 is the drop-in; F32/F32 the FP32 storage variant).
+ * PLANAR: values and embedded x in two separate slot-ordered arrays (slot s of
+ *   tile T, brick-row R, lane C is T*tileH*8 + R*8 + C); used for mixed types
+ *   such as BF16 values with FP32 x. */
+enum { INTSPMV_AXT_INTERLEAVED = 0, INTSPMV_AXT_PLANAR = 1 };
+
+/* con_host: str_matAXT.con (lenCON = nrows) from the builder. */
+int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc, const uint32_t *con_host,
+                     intspmv_plan **plan);
+/* Embedded-x product.  INTERLEAVED: ax = the reference array, xplane = NULL.
+ * PLANAR: ax = value plane, xplane = x plane.  y has the accumulation type
+ * (double for F64 values, float otherwise). */
+int spmv_axt(intspmv_ctx *ctx, const intspmv_plan *plan, const void *ax, const void *xplane,
+             const uint32_t *sec, void *y);
+
+int intspmv_plan_destroy(intspmv_plan *plan);
+/* number of work blocks / split rows of a plan (for reporting) */
+int intspmv_plan_info(const intspmv_plan *plan, uint32_t *nblocks, uint32_t *nsplit_rows,
+                      uint32_t *nlong_rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INTSPMV_H */

@@ -1,0 +1,264 @@
+// abi.hip -- context, memory, error and plan entry points of libintspmv.
+//
+// The plan is the host-side analysis that replaces the reference's per-call
+// OpenMP dynamic scheduling (schedule(dynamic) over rows / tiles / chunks,
+// intSpmv_main.c:717, 948, 1391, 1451): rows are packed once into work blocks
+// of bounded size, one per wave, so every launch is a static, balanced sweep.
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "common.hpp"
+
+extern "C" const char *intspmv_strerror(int st)
+{
+    switch (st) {
+    case INTSPMV_OK: return "ok";
+    case INTSPMV_ERR_ARG: return "invalid argument";
+    case INTSPMV_ERR_HIP: return "HIP runtime error";
+    case INTSPMV_ERR_NOMEM: return "out of memory";
+    case INTSPMV_ERR_UNSUPPORTED: return "unsupported combination";
+    case INTSPMV_ERR_OVERFLOW: return "size exceeds the 32-bit index range";
+    case INTSPMV_ERR_NODEVICE: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+int ispv_enter(intspmv_ctx *ctx)
+{
+    if (!ctx) return INTSPMV_ERR_ARG;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != ctx->device) e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        return INTSPMV_ERR_HIP;
+    }
+    return INTSPMV_OK;
+}
+
+int ispv_launch_check(intspmv_ctx *ctx)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        return INTSPMV_ERR_HIP;
+    }
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_device_count(int *count)
+{
+    if (!count) return INTSPMV_ERR_ARG;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return INTSPMV_ERR_NODEVICE;
+    }
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_create(intspmv_ctx **ctx, int device)
+{
+    if (!ctx) return INTSPMV_ERR_ARG;
+    *ctx = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return INTSPMV_ERR_NODEVICE;
+    if (device < 0 || device >= n) return INTSPMV_ERR_ARG;
+    intspmv_ctx *c = (intspmv_ctx *)calloc(1, sizeof(intspmv_ctx));
+    if (!c) return INTSPMV_ERR_NOMEM;
+    c->device = device;
+    c->stream = nullptr;
+    if (hipSetDevice(device) != hipSuccess) {
+        free(c);
+        return INTSPMV_ERR_HIP;
+    }
+    *ctx = c;
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_destroy(intspmv_ctx *ctx)
+{
+    free(ctx);
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_set_stream(intspmv_ctx *ctx, void *stream)
+{
+    if (!ctx) return INTSPMV_ERR_ARG;
+    ctx->stream = (hipStream_t)stream;
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_sync(intspmv_ctx *ctx)
+{
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        return INTSPMV_ERR_HIP;
+    }
+    return INTSPMV_OK;
+}
+
+extern "C" const char *intspmv_last_hip_error(intspmv_ctx *ctx)
+{
+    return ctx ? hipGetErrorString(ctx->last_err) : "no context";
+}
+
+extern "C" int intspmv_malloc(intspmv_ctx *ctx, void **dptr, size_t bytes)
+{
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    if (!dptr) return INTSPMV_ERR_ARG;
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        *dptr = nullptr;
+        return INTSPMV_ERR_NOMEM;
+    }
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_free(intspmv_ctx *ctx, void *dptr)
+{
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    if (dptr) {
+        hipError_t e = hipFree(dptr);
+        if (e != hipSuccess) {
+            ctx->last_err = e;
+            return INTSPMV_ERR_HIP;
+        }
+    }
+    return INTSPMV_OK;
+}
+
+static int copy(intspmv_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind k)
+{
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    if (bytes == 0) return INTSPMV_OK;
+    if (!dst || !src) return INTSPMV_ERR_ARG;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, k, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        return INTSPMV_ERR_HIP;
+    }
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_memcpy_htod(intspmv_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    return copy(ctx, dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+extern "C" int intspmv_memcpy_dtoh(intspmv_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    return copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost);
+}
+
+// Packs rows into blocks.  uptr[r]..uptr[r+1] are row r's units (monotone,
+// n+1 entries).  Rows with more than long_units units become long blocks,
+// split into ceil(len / long_piece) near-equal pieces when longer than
+// long_piece; other rows are packed greedily, at most umax units and rmax
+// rows per block.
+int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
+                    uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_piece,
+                    intspmv_plan **out)
+{
+    if (!out || !uptr || umax == 0 || rmax == 0 || long_units >= umax || long_piece == 0)
+        return INTSPMV_ERR_ARG;
+    *out = nullptr;
+    for (uint32_t r = 0; r < nrows; r++)
+        if (uptr[r + 1] < uptr[r]) return INTSPMV_ERR_ARG;
+    std::vector<ispv::BlockDesc> blocks;
+    uint32_t nsplit = 0, nlong = 0;
+    uint32_t r = 0;
+    while (r < nrows) {
+        uint32_t len = uptr[r + 1] - uptr[r];
+        if (len > long_units) {
+            uint32_t np = (len + long_piece - 1) / long_piece;
+            uint32_t b0 = (uint32_t)blocks.size();
+            if (np == 1) {
+                blocks.push_back({uptr[r], uptr[r + 1], r, r + 1, b0, 1, 0, 0});
+                nlong++;
+            } else {
+                uint64_t per = (len + (uint64_t)np - 1) / np;
+                for (uint32_t p = 0; p < np; p++) {
+                    uint64_t a = uptr[r] + p * per, b = a + per;
+                    if (b > uptr[r + 1]) b = uptr[r + 1];
+                    blocks.push_back({(uint32_t)a, (uint32_t)b, r, r + 1, b0, np, 0, 0});
+                }
+                nsplit++;
+            }
+            r++;
+            continue;
+        }
+        uint32_t r0 = r, units = 0;
+        while (r < nrows && r - r0 < rmax) {
+            uint32_t l = uptr[r + 1] - uptr[r];
+            if (l > long_units || units + l > umax) break;
+            units += l;
+            r++;
+        }
+        blocks.push_back({uptr[r0], uptr[r], r0, r, 0, 0, 0, 0});
+    }
+    intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));
+    if (!p) return INTSPMV_ERR_NOMEM;
+    p->ctx = ctx;
+    p->kind = kind;
+    p->nrows = nrows;
+    p->nunits = nrows ? uptr[nrows] : 0;
+    p->nblocks = (uint32_t)blocks.size();
+    p->nsplit_rows = nsplit;
+    p->nlong_rows = nlong;
+    p->umax = umax;
+    p->rmax = rmax;
+    p->long_units = long_units;
+    size_t nb = blocks.size() ? blocks.size() : 1;
+    int st = ispv_enter(ctx);
+    if (st) {
+        free(p);
+        return st;
+    }
+    hipError_t e = hipMalloc(&p->d_blocks, nb * sizeof(ispv::BlockDesc));
+    if (e == hipSuccess) e = hipMalloc(&p->d_partial, nb * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&p->d_cnt, nb * sizeof(uint32_t));
+    if (e == hipSuccess && !blocks.empty())
+        e = hipMemcpy(p->d_blocks, blocks.data(), blocks.size() * sizeof(ispv::BlockDesc),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(p->d_cnt, 0, nb * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        intspmv_plan_destroy(p);
+        return INTSPMV_ERR_NOMEM;
+    }
+    *out = p;
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_plan_destroy(intspmv_plan *p)
+{
+    if (!p) return INTSPMV_OK;
+    if (p->ctx) ispv_enter(p->ctx);
+    if (p->d_blocks) (void)hipFree(p->d_blocks);
+    if (p->d_partial) (void)hipFree(p->d_partial);
+    if (p->d_cnt) (void)hipFree(p->d_cnt);
+    free(p);
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_plan_info(const intspmv_plan *p, uint32_t *nblocks, uint32_t *nsplit,
+                                 uint32_t *nlong)
+{
+    if (!p) return INTSPMV_ERR_ARG;
+    if (nblocks) *nblocks = p->nblocks;
+    if (nsplit) *nsplit = p->nsplit_rows;
+    if (nlong) *nlong = p->nlong_rows;
+    return INTSPMV_OK;
+}

@@ -1,0 +1,265 @@
+// axc_k1.hip -- AXC and K1 kernels.
+//
+// AXC (int_axc, intSpmv_main.c:711-732): rows of 16-element bricks
+//   [8 values | 8 embedded x] starting at brp[r].  Phase 1 forms each brick's
+//   reduce tree with four lanes per brick (as the AXT th==1 kernel); phase 2
+//   gives each row to one lane, which adds its brick sums in order from +0.0
+//   -- the reference's `sum = sum + red`.
+// K1 (int_k1 / int_k1p, intSpmv_main.c:943-1066): chunk c holds 8 rows of the
+//   length-sorted order, element k of lane l at chp[c] + 8k + l.  Eight lanes
+//   per chunk, one fma chain per lane in k order (the reference's lane-wise
+//   vtSum).  Chunks with more than K1_LONG steps are cut into pieces that
+//   meet in the plan scratch.  The result goes to res[8c + l] (int_k1) or is
+//   scattered to y[permi[8c + l]] for the valid lanes (int_k1p).
+#include <stdlib.h>
+
+#include <vector>
+
+#include "common.hpp"
+#include "reduce.hpp"
+
+namespace ispv {
+
+constexpr uint32_t AXC_UMAX = 256;
+constexpr uint32_t AXC_RMAX = 256;
+constexpr uint32_t AXC_LONG = 64;
+constexpr uint32_t K1_LONG = 512; // steps per K1 work item
+
+__global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
+    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const double *__restrict__ ax,
+    const uint32_t *__restrict__ brp, double *__restrict__ y, uint32_t nrows, double *partial,
+    uint32_t *cnt)
+{
+    __shared__ double s_val[WAVES][AXC_UMAX];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    if (bid >= nblocks) return;
+    const BlockDesc d = blocks[bid];
+    const uint32_t nu = d.uend - d.ubeg, q = lane & 3;
+    constexpr int U = 4;
+    for (uint32_t base = 0; base < nu; base += 16 * U) {
+        double2 v[U], xx[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            uint32_t t = base + j * 16 + lane / 4;
+            const double *b = ax + ((size_t)d.ubeg + (t < nu ? t : nu - 1)) * 16 + 2 * q;
+            v[j] = *reinterpret_cast<const double2 *>(b);
+            xx[j] = *reinterpret_cast<const double2 *>(b + 8);
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const bool ok = base + j * 16 + lane / 4 < nu;
+            double p0 = ok ? v[j].x * xx[j].x : 0.0, p1 = ok ? v[j].y * xx[j].y : 0.0;
+            p0 = p0 + dpp_f64<DPP_QUAD_XOR2>(p0);
+            p1 = p1 + dpp_f64<DPP_QUAD_XOR2>(p1);
+            p0 = p0 + dpp_f64<DPP_QUAD_XOR1>(p0);
+            p1 = p1 + dpp_f64<DPP_QUAD_XOR1>(p1);
+            uint32_t t = base + j * 16 + lane / 4;
+            if (t < nu && q == 0) s_val[w][t] = p0 + p1;
+        }
+    }
+    wave_lds_sync();
+    if (d.k != 0) {
+        reduce_long<double>(s_val[w], nu, d, bid, y, nrows, partial, cnt);
+        return;
+    }
+    for (uint32_t r = d.rbeg + lane; r < d.rend; r += WAVE) {
+        uint32_t a = brp[r] / 16 - d.ubeg, b = brp[r + 1] / 16 - d.ubeg;
+        double sum = 0.0;
+        for (uint32_t i = a; i < b; i++) sum = sum + s_val[w][i];
+        y[r] = sum;
+    }
+}
+
+// K1 work item: chunk c, steps [kbeg, kend); k > 1 marks pieces b0..b0+k-1.
+struct K1Item {
+    uint32_t c, kbeg, kend, b0, k, pad0, pad1, pad2;
+};
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k1_kernel(
+    const K1Item *__restrict__ items, uint32_t nitems, const double *__restrict__ val,
+    const uint32_t *__restrict__ col, const uint32_t *__restrict__ chp,
+    const uint32_t *__restrict__ permi, const double *__restrict__ x, double *__restrict__ y,
+    uint32_t nrows, double *partial, uint32_t *cnt)
+{
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const uint32_t wid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    const uint32_t it = wid * 8 + lane / 8, l = lane % 8;
+    bool have = it < nitems;
+    K1Item d = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (have) d = items[it];
+    double acc = 0.0;
+    if (have) {
+        const uint32_t base = chp[d.c] + l;
+        uint32_t k = d.kbeg;
+        for (; k + 4 <= d.kend; k += 4) {
+            double v[4];
+            uint32_t cc[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                v[j] = val[base + 8 * (k + j)];
+                cc[j] = col[base + 8 * (k + j)];
+            }
+            double xv[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) xv[j] = x[cc[j]];
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc = fma(v[j], xv[j], acc);
+        }
+        for (; k < d.kend; k++) acc = fma(val[base + 8 * k], x[col[base + 8 * k]], acc);
+    }
+    // write or combine
+    const uint32_t s = d.c * 8 + l;
+    if (have && d.k <= 1) {
+        if (!permi) y[s] = acc;
+        else if (s < nrows) y[permi[s]] = acc;
+    }
+    // split pieces only: publish, count, and let the last piece combine
+    if (!__any(have && d.k > 1)) return;
+    if (have && d.k > 1) {
+        __hip_atomic_store(&partial[(size_t)it * 8 + l], acc, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    uint32_t old = 0;
+    if (have && d.k > 1 && l == 0)
+        old = __hip_atomic_fetch_add(&cnt[d.b0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    old = __shfl(old, lane & ~7, WAVE);
+    if (have && d.k > 1 && old == d.k - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        double tot = 0.0;
+        for (uint32_t j = 0; j < d.k; j++)
+            tot = tot + __hip_atomic_load(&partial[(size_t)(d.b0 + j) * 8 + l], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        if (!permi) y[s] = tot;
+        else if (s < nrows) y[permi[s]] = tot;
+        if (l == 0) __hip_atomic_store(&cnt[d.b0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k1_order_kernel(uint32_t nrows, const uint32_t *__restrict__ permi,
+                                const double *__restrict__ res, double *__restrict__ y)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrows) y[permi[i]] = res[i];
+}
+
+} // namespace ispv
+
+using namespace ispv;
+
+extern "C" int intspmv_axc_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *brp_host,
+                                intspmv_plan **plan)
+{
+    if (!ctx || !plan || !brp_host || brp_host[0] != 0) return INTSPMV_ERR_ARG;
+    uint32_t *uptr = (uint32_t *)malloc(((size_t)nrows + 1) * sizeof(uint32_t));
+    if (!uptr) return INTSPMV_ERR_NOMEM;
+    for (uint32_t r = 0; r <= nrows; r++) {
+        if (brp_host[r] % 16) {
+            free(uptr);
+            return INTSPMV_ERR_ARG;
+        }
+        uptr[r] = brp_host[r] / 16;
+    }
+    int st = ispv_plan_build(ctx, PLAN_AXC, nrows, uptr, AXC_UMAX, AXC_RMAX, AXC_LONG, AXC_UMAX,
+                             plan);
+    free(uptr);
+    return st;
+}
+
+extern "C" int spmv_axc(intspmv_ctx *ctx, const intspmv_plan *plan, const double *ax,
+                        const uint32_t *brp, double *y)
+{
+    if (!plan || plan->kind != PLAN_AXC || plan->ctx != ctx) return INTSPMV_ERR_ARG;
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    if (plan->nblocks == 0) return INTSPMV_OK;
+    if (!brp || !y || (plan->nunits && !ax)) return INTSPMV_ERR_ARG;
+    dim3 grid((plan->nblocks + WAVES - 1) / WAVES);
+    axc_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(plan->d_blocks, plan->nblocks, ax, brp,
+                                                        y, plan->nrows, plan->d_partial,
+                                                        plan->d_cnt);
+    return ispv_launch_check(ctx);
+}
+
+extern "C" int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkNum,
+                               const uint32_t *nmc_host, intspmv_plan **plan)
+{
+    if (!ctx || !plan || (chunkNum && !nmc_host)) return INTSPMV_ERR_ARG;
+    if (chunkNum != (nrows + 7) / 8) return INTSPMV_ERR_ARG;
+    *plan = nullptr;
+    std::vector<K1Item> items;
+    uint32_t nsplit = 0;
+    for (uint32_t c = 0; c < chunkNum; c++) {
+        uint32_t n = nmc_host[c];
+        if (n <= K1_LONG) {
+            items.push_back({c, 0, n, (uint32_t)items.size(), 1, 0, 0, 0});
+            continue;
+        }
+        uint32_t np = (n + K1_LONG - 1) / K1_LONG, per = (n + np - 1) / np;
+        uint32_t b0 = (uint32_t)items.size();
+        for (uint32_t p = 0; p < np; p++) {
+            uint32_t a = p * per, b = a + per > n ? n : a + per;
+            items.push_back({c, a, b, b0, np, 0, 0, 0});
+        }
+        nsplit++;
+    }
+    intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));
+    if (!p) return INTSPMV_ERR_NOMEM;
+    p->ctx = ctx;
+    p->kind = PLAN_K1;
+    p->nrows = nrows;
+    p->chunkNum = chunkNum;
+    p->nblocks = (uint32_t)items.size();
+    p->nsplit_rows = nsplit;
+    int st = ispv_enter(ctx);
+    if (st) {
+        free(p);
+        return st;
+    }
+    size_t ni = items.empty() ? 1 : items.size();
+    hipError_t e = hipMalloc(&p->d_blocks, ni * sizeof(K1Item));
+    if (e == hipSuccess) e = hipMalloc(&p->d_partial, ni * 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&p->d_cnt, ni * sizeof(uint32_t));
+    if (e == hipSuccess && !items.empty())
+        e = hipMemcpy(p->d_blocks, items.data(), items.size() * sizeof(K1Item),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(p->d_cnt, 0, ni * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        intspmv_plan_destroy(p);
+        return INTSPMV_ERR_NOMEM;
+    }
+    *plan = p;
+    return INTSPMV_OK;
+}
+
+extern "C" int spmv_k1(intspmv_ctx *ctx, const intspmv_plan *plan, const double *val,
+                       const uint32_t *col, const uint32_t *chp, const uint32_t *nmc,
+                       const uint32_t *permi, const double *x, double *y)
+{
+    (void)nmc; // lengths are carried by the plan's work items
+    if (!plan || plan->kind != PLAN_K1 || plan->ctx != ctx) return INTSPMV_ERR_ARG;
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    if (plan->nblocks == 0) return INTSPMV_OK;
+    if (!chp || !x || !y || !val || !col) return INTSPMV_ERR_ARG;
+    uint32_t waves = (plan->nblocks + 7) / 8;
+    dim3 grid((waves + WAVES - 1) / WAVES);
+    k1_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(
+        (const K1Item *)plan->d_blocks, plan->nblocks, val, col, chp, permi, x, y, plan->nrows,
+        plan->d_partial, plan->d_cnt);
+    return ispv_launch_check(ctx);
+}
+
+extern "C" int spmv_k1_order(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *permi,
+                             const double *res, double *y)
+{
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    if (nrows == 0) return INTSPMV_OK;
+    if (!permi || !res || !y) return INTSPMV_ERR_ARG;
+    k1_order_kernel<<<(nrows + 255) / 256, 256, 0, ctx->stream>>>(nrows, permi, res, y);
+    return ispv_launch_check(ctx);
+}

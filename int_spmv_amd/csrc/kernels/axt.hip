@@ -1,0 +1,313 @@
+// axt.hip -- AXT NOC kernels (embedded x) and the spmv_axt entry point.
+//
+// Reference kernels replaced: int_axt_noc_h1 (intSpmv_main.c:1385-1404) and
+// int_axt_noc (intSpmv_main.c:1443-1474).  Layout (getArraysAxSecAXT_NOC*,
+// intSpmv_main.c:1152-1221): tile T of height th holds th brick-rows of
+// 8 values followed by the 8 embedded x values; lane C of the tile is a
+// column; for th == 1 sec[T] is the row of the whole tile, for th > 1
+// sec[8T+C] is the row of column C.
+//
+// Wave mapping on gfx950: four lanes per tile, each lane streaming two
+// adjacent lanes of the tile (16-byte loads of values and of x for FP64), so a
+// wave covers 16 tiles per step with fully used 64-byte runs.
+//   th == 1: the tile sum is the reference's reduce tree
+//            ((p0+p4)+(p2+p6)) + ((p1+p5)+(p3+p7)), formed with two DPP
+//            quad exchanges (xor 2, xor 1).
+//   th  > 1: each lane keeps two column sums, one fused multiply-add per
+//            brick-row, in brick-row order (the reference's lane-wise
+//            vertical sum).
+// Unit sums go to LDS and phase 2 (reduce.hpp) forms the rows.
+#include "common.hpp"
+#include "reduce.hpp"
+
+namespace ispv {
+
+constexpr uint32_t AXT_UMAX = 256;  // units (tiles or columns) per block
+constexpr uint32_t AXT_RMAX = 256;  // rows per block
+
+template <typename T>
+struct Pair;
+template <>
+struct Pair<double> {
+    __device__ __forceinline__ static void ld(const double *p, double &a, double &b)
+    {
+        double2 v = *reinterpret_cast<const double2 *>(p);
+        a = v.x;
+        b = v.y;
+    }
+};
+template <>
+struct Pair<float> {
+    __device__ __forceinline__ static void ld(const float *p, float &a, float &b)
+    {
+        float2 v = *reinterpret_cast<const float2 *>(p);
+        a = v.x;
+        b = v.y;
+    }
+};
+template <>
+struct Pair<uint16_t> { // bf16 storage
+    __device__ __forceinline__ static void ld(const uint16_t *p, float &a, float &b)
+    {
+        uint32_t v = *reinterpret_cast<const uint32_t *>(p);
+        a = __uint_as_float(v << 16);
+        b = __uint_as_float(v & 0xFFFF0000u);
+    }
+};
+
+template <typename VT>
+using LoadT = typename std::conditional<std::is_same<VT, uint16_t>::value, float, VT>::type;
+
+template <typename AT>
+__device__ __forceinline__ AT fma_acc(AT a, AT b, AT c);
+template <>
+__device__ __forceinline__ double fma_acc<double>(double a, double b, double c) { return fma(a, b, c); }
+template <>
+__device__ __forceinline__ float fma_acc<float>(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <typename AT>
+__device__ __forceinline__ AT quad_xor2(AT v);
+template <>
+__device__ __forceinline__ double quad_xor2<double>(double v) { return dpp_f64<DPP_QUAD_XOR2>(v); }
+template <>
+__device__ __forceinline__ float quad_xor2<float>(float v) { return dpp_f32<DPP_QUAD_XOR2>(v); }
+template <typename AT>
+__device__ __forceinline__ AT quad_xor1(AT v);
+template <>
+__device__ __forceinline__ double quad_xor1<double>(double v) { return dpp_f64<DPP_QUAD_XOR1>(v); }
+template <>
+__device__ __forceinline__ float quad_xor1<float>(float v) { return dpp_f32<DPP_QUAD_XOR1>(v); }
+
+// Element index of value slot (T, R, C) and of its x; interleaved: one array
+// with 16 elements per brick-row, planar: two arrays with 8 per brick-row.
+template <bool PLANAR>
+__device__ __forceinline__ size_t vidx(size_t T, uint32_t th, uint32_t R, uint32_t C)
+{
+    return PLANAR ? (T * th + R) * 8 + C : (T * th + R) * 16 + C;
+}
+template <bool PLANAR>
+__device__ __forceinline__ size_t xidx(size_t T, uint32_t th, uint32_t R, uint32_t C)
+{
+    return PLANAR ? (T * th + R) * 8 + C : (T * th + R) * 16 + 8 + C;
+}
+
+template <typename VT, typename XT, typename AT, bool PLANAR>
+__global__ __launch_bounds__(BLOCK_THREADS) void axt_h1_kernel(
+    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ av,
+    const XT *__restrict__ axv, const uint32_t *__restrict__ sec, AT *__restrict__ y,
+    uint32_t nrows, double *partial, uint32_t *cnt)
+{
+    __shared__ AT s_val[WAVES][AXT_UMAX];
+    __shared__ uint32_t s_key[WAVES][AXT_UMAX];
+    __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    if (bid >= nblocks) return;
+    const BlockDesc d = blocks[bid];
+    const uint32_t nu = d.uend - d.ubeg;
+    const uint32_t q = lane & 3, C = 2 * q;
+    constexpr int U = 4; // steps of 16 tiles in flight per lane
+    for (uint32_t base = 0; base < nu; base += 16 * U) {
+        LoadT<VT> v0[U], v1[U];
+        LoadT<XT> x0[U], x1[U];
+        uint32_t key[U];
+        // branch-free loads: lanes past the block re-read its last tile and
+        // are masked afterwards, so all loads of the batch issue back to back
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            uint32_t t = base + j * 16 + lane / 4;
+            size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+            Pair<VT>::ld(av + vidx<PLANAR>(T, 1, 0, C), v0[j], v1[j]);
+            Pair<XT>::ld((PLANAR ? axv : (const XT *)av) + xidx<PLANAR>(T, 1, 0, C), x0[j], x1[j]);
+            key[j] = sec[T];
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const bool ok = base + j * 16 + lane / 4 < nu;
+            AT p0 = ok ? (AT)v0[j] * (AT)x0[j] : (AT)0, p1 = ok ? (AT)v1[j] * (AT)x1[j] : (AT)0;
+            p0 = p0 + quad_xor2<AT>(p0);
+            p1 = p1 + quad_xor2<AT>(p1);
+            p0 = p0 + quad_xor1<AT>(p0);
+            p1 = p1 + quad_xor1<AT>(p1);
+            AT s = p0 + p1;
+            uint32_t t = base + j * 16 + lane / 4;
+            if (t < nu && q == 0) {
+                s_val[w][t] = s;
+                s_key[w][t] = key[j];
+            }
+        }
+    }
+    wave_lds_sync();
+    reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial, cnt);
+}
+
+template <typename VT, typename XT, typename AT, bool PLANAR, int TH>
+__global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
+    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ av,
+    const XT *__restrict__ axv, const uint32_t *__restrict__ sec, AT *__restrict__ y,
+    uint32_t nrows, double *partial, uint32_t *cnt, uint32_t th_rt)
+{
+    __shared__ AT s_val[WAVES][AXT_UMAX];
+    __shared__ uint32_t s_key[WAVES][AXT_UMAX];
+    __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
+    const uint32_t th = TH ? TH : th_rt;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    if (bid >= nblocks) return;
+    const BlockDesc d = blocks[bid];
+    const uint32_t nu = d.uend - d.ubeg;
+    const uint32_t C = 2 * (lane & 3);
+    const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
+    for (uint32_t Tb = T0; Tb < T1; Tb += 16) {
+        const uint32_t Tr = Tb + lane / 4, u0 = 8 * Tr + C;
+        const bool ok0 = u0 >= d.ubeg && u0 < d.uend;
+        const bool ok1 = u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
+        // lanes past the block re-read its last tile (results discarded) so
+        // that the loads stay branch-free
+        const uint32_t T = Tr < T1 ? Tr : T1 - 1;
+        AT a0 = 0, a1 = 0;
+        uint32_t k0 = 0, k1 = 0;
+        {
+            const XT *xb = PLANAR ? axv : (const XT *)av;
+            if constexpr (TH > 0) {
+                LoadT<VT> v0[TH], v1[TH];
+                LoadT<XT> x0[TH], x1[TH];
+#pragma unroll
+                for (int R = 0; R < TH; R++) {
+                    Pair<VT>::ld(av + vidx<PLANAR>(T, TH, R, C), v0[R], v1[R]);
+                    Pair<XT>::ld(xb + xidx<PLANAR>(T, TH, R, C), x0[R], x1[R]);
+                }
+#pragma unroll
+                for (int R = 0; R < TH; R++) {
+                    a0 = fma_acc<AT>((AT)v0[R], (AT)x0[R], a0);
+                    a1 = fma_acc<AT>((AT)v1[R], (AT)x1[R], a1);
+                }
+            } else {
+                for (uint32_t R = 0; R < th; R++) {
+                    LoadT<VT> v0, v1;
+                    LoadT<XT> x0, x1;
+                    Pair<VT>::ld(av + vidx<PLANAR>(T, th, R, C), v0, v1);
+                    Pair<XT>::ld(xb + xidx<PLANAR>(T, th, R, C), x0, x1);
+                    a0 = fma_acc<AT>((AT)v0, (AT)x0, a0);
+                    a1 = fma_acc<AT>((AT)v1, (AT)x1, a1);
+                }
+            }
+            uint2 kk = *reinterpret_cast<const uint2 *>(sec + 8 * (size_t)T + C);
+            k0 = kk.x;
+            k1 = kk.y;
+        }
+        if (ok0) {
+            s_val[w][u0 - d.ubeg] = a0;
+            s_key[w][u0 - d.ubeg] = k0;
+        }
+        if (ok1) {
+            s_val[w][u0 + 1 - d.ubeg] = a1;
+            s_key[w][u0 + 1 - d.ubeg] = k1;
+        }
+    }
+    wave_lds_sync();
+    reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial, cnt);
+}
+
+template <typename VT, typename XT, typename AT, bool PLANAR>
+static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, const uint32_t *sec,
+                      void *y)
+{
+    const uint32_t nb = p->nblocks, th = p->axt.tileH;
+    if (nb == 0) return INTSPMV_OK;
+    dim3 grid((nb + WAVES - 1) / WAVES), blk(BLOCK_THREADS);
+    hipStream_t s = p->ctx->stream;
+    auto a = (const VT *)ax;
+    auto x = (const XT *)xp;
+    auto yy = (AT *)y;
+    if (th == 1) {
+        axt_h1_kernel<VT, XT, AT, PLANAR><<<grid, blk, 0, s>>>(
+            p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, p->d_cnt);
+    } else {
+#define ISPV_TH(N)                                                                            \
+    case N:                                                                                   \
+        axt_noc_kernel<VT, XT, AT, PLANAR, N><<<grid, blk, 0, s>>>(                           \
+            p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, p->d_cnt, th);            \
+        break;
+        switch (th) {
+            ISPV_TH(2)
+            ISPV_TH(4)
+            ISPV_TH(8)
+            ISPV_TH(16)
+        default:
+            axt_noc_kernel<VT, XT, AT, PLANAR, 0><<<grid, blk, 0, s>>>(
+                p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, p->d_cnt, th);
+        }
+#undef ISPV_TH
+    }
+    return ispv_launch_check(p->ctx);
+}
+
+} // namespace ispv
+
+using namespace ispv;
+
+extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
+                                const uint32_t *con_host, intspmv_plan **plan)
+{
+    if (!ctx || !desc || !plan || (desc->nrows && !con_host)) return INTSPMV_ERR_ARG;
+    const intspmv_axt_desc &D = *desc;
+    if (D.tileHW != 8 || D.tileH == 0) return INTSPMV_ERR_ARG;
+    if (D.mode != INTSPMV_NOC) return INTSPMV_ERR_UNSUPPORTED;
+    const uint32_t th = D.tileH, ths = 8 * th;
+    // unit start of each row from con (getArraysLenAXT, intSpmv_main.c:1130-1134):
+    // con = T*2ths + R*16 + C; NOC rows start at brick-row 0 of a column.
+    const uint32_t nunits = (th == 1) ? D.colN / 8 : D.colN;
+    if (th == 1 && (D.colN % 8 != 0 || nunits != D.tileN)) return INTSPMV_ERR_ARG;
+    if (th > 1 && (D.colN > D.tileN * 8 || D.lenSEC != D.tileN * 8)) return INTSPMV_ERR_ARG;
+    if ((uint64_t)D.tileN * 2 * ths != D.lenAX) return INTSPMV_ERR_ARG;
+    uint32_t *uptr = (uint32_t *)malloc(((size_t)D.nrows + 1) * sizeof(uint32_t));
+    if (!uptr) return INTSPMV_ERR_NOMEM;
+    for (uint32_t r = 0; r < D.nrows; r++) {
+        uint32_t c = con_host[r], T = c / (2 * ths), w = c % (2 * ths);
+        uint32_t col = w % 16;
+        uptr[r] = (th == 1) ? T : T * 8 + col;
+        if (col >= 8 || w / 16 != 0 || uptr[r] > nunits || (r && uptr[r] < uptr[r - 1])) {
+            free(uptr);
+            return INTSPMV_ERR_ARG;
+        }
+    }
+    uptr[D.nrows] = nunits;
+    if (D.nrows && uptr[D.nrows - 1] > nunits) {
+        free(uptr);
+        return INTSPMV_ERR_ARG;
+    }
+    uint32_t long_units = th == 1 ? 64 : (512 / th < 4 ? 4 : (512 / th > 64 ? 64 : 512 / th));
+    int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, AXT_UMAX, AXT_RMAX, long_units,
+                             AXT_UMAX, plan);
+    free(uptr);
+    if (st) return st;
+    (*plan)->axt = D;
+    return INTSPMV_OK;
+}
+
+extern "C" int spmv_axt(intspmv_ctx *ctx, const intspmv_plan *plan, const void *ax,
+                        const void *xplane, const uint32_t *sec, void *y)
+{
+    if (!ctx || !plan || plan->kind != PLAN_AXT || plan->ctx != ctx) return INTSPMV_ERR_ARG;
+    const intspmv_axt_desc &D = plan->axt;
+    if (plan->nrows && (!y || (plan->nunits && (!ax || !sec)))) return INTSPMV_ERR_ARG;
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    const bool planar = D.layout == INTSPMV_AXT_PLANAR;
+    if (planar && plan->nunits && !xplane) return INTSPMV_ERR_ARG;
+    if (!planar) {
+        if (D.val_type == INTSPMV_F64 && D.x_type == INTSPMV_F64)
+            return launch_noc<double, double, double, false>(plan, ax, nullptr, sec, y);
+        if (D.val_type == INTSPMV_F32 && D.x_type == INTSPMV_F32)
+            return launch_noc<float, float, float, false>(plan, ax, nullptr, sec, y);
+        return INTSPMV_ERR_UNSUPPORTED;
+    }
+    if (D.val_type == INTSPMV_F64 && D.x_type == INTSPMV_F64)
+        return launch_noc<double, double, double, true>(plan, ax, xplane, sec, y);
+    if (D.val_type == INTSPMV_F32 && D.x_type == INTSPMV_F32)
+        return launch_noc<float, float, float, true>(plan, ax, xplane, sec, y);
+    if (D.val_type == INTSPMV_BF16 && D.x_type == INTSPMV_F32)
+        return launch_noc<uint16_t, float, float, true>(plan, ax, xplane, sec, y);
+    return INTSPMV_ERR_UNSUPPORTED;
+}

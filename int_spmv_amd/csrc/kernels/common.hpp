@@ -1,0 +1,149 @@
+// common.hpp -- shared device helpers and host-side structures of libintspmv.
+//
+// Execution model (all kernels): a workgroup is WAVES wave64 wavefronts; each
+// wave owns one work block of the plan and never synchronises with the other
+// waves of its workgroup, so per-wave LDS regions are private and only
+// wave-level ordering fences are used.  Work blocks are handed out so that
+// the eight XCDs of the MI355X each stream a contiguous range of blocks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "intspmv.h"
+
+namespace ispv {
+
+constexpr int WAVE = 64;
+constexpr int WAVES = 4;            // waves per workgroup
+constexpr int BLOCK_THREADS = WAVE * WAVES;
+constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// One unit of work for one wave.  Units are the format's reduction atoms
+// (CSR entries, AXC bricks, AXT tiles for th==1, AXT tile columns for th>1,
+// AXT COM slots).  Rows [rbeg, rend) own units [ubeg, uend).
+//   k == 0 : normal block, whole rows, each summed by one lane in unit order;
+//   k == 1 : one long row, reduced by a wave tree, written directly;
+//   k >= 2 : piece of a long row split over k consecutive blocks b0..b0+k-1;
+//            partial sums meet in the plan's scratch, the last wave to arrive
+//            adds them in piece order.
+struct BlockDesc {
+    uint32_t ubeg, uend, rbeg, rend;
+    uint32_t b0, k, pad0, pad1;
+};
+
+// Maps the flat workgroup id so that XCD x (which receives ids x, x+8, ...)
+// gets a contiguous range [x*q + min(x, r), ...) of logical workgroups.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t g, uint32_t n)
+{
+    uint32_t q = n / NUM_XCD, r = n % NUM_XCD, x = g % NUM_XCD, l = g / NUM_XCD;
+    return x * q + (x < r ? x : r) + l;
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Orders this wave's LDS accesses before/after the call (no cross-wave sync).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// ---- cross-lane moves (DPP within a quad, ds_bpermute otherwise) ----------
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+constexpr int DPP_QUAD_XOR1 = 0xB1; // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_XOR2 = 0x4E; // quad_perm [2,3,0,1]
+
+__device__ __forceinline__ double shfl_xor_f64(double v, int m) { return __shfl_xor(v, m, WAVE); }
+__device__ __forceinline__ float shfl_xor_f32(float v, int m) { return __shfl_xor(v, m, WAVE); }
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+    v += dpp_f64<DPP_QUAD_XOR1>(v);
+    v += dpp_f64<DPP_QUAD_XOR2>(v);
+#pragma unroll
+    for (int m = 4; m < WAVE; m <<= 1) v += shfl_xor_f64(v, m);
+    return v;
+}
+__device__ __forceinline__ float wave_sum_f32(float v)
+{
+    v += dpp_f32<DPP_QUAD_XOR1>(v);
+    v += dpp_f32<DPP_QUAD_XOR2>(v);
+#pragma unroll
+    for (int m = 4; m < WAVE; m <<= 1) v += shfl_xor_f32(v, m);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+// ---- bf16 ----------------------------------------------------------------
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// ---- split-row combination ----------------------------------------------
+// Publishes this block's partial and, when it is the last of the k pieces of
+// its row to arrive, returns true with the ordered sum of all k partials in
+// *total.  No wave ever waits for another: the last arriver does the sum.
+// Counters reset themselves, so the scratch needs no clearing between calls.
+__device__ __forceinline__ bool split_publish(double part, uint32_t blk, uint32_t b0, uint32_t k,
+                                             double *partial, uint32_t *cnt, double *total)
+{
+    bool last = false;
+    if (lane_id() == 0) {
+        __hip_atomic_store(&partial[blk], part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t old = __hip_atomic_fetch_add(&cnt[b0], 1u, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        if (old == k - 1) {
+            double s = 0.0;
+            for (uint32_t j = 0; j < k; j++)
+                s = s + __hip_atomic_load(&partial[b0 + j], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&cnt[b0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *total = s;
+            last = true;
+        }
+    }
+    return last;
+}
+
+} // namespace ispv
+
+// ---- host-side objects -----------------------------------------------------
+struct intspmv_ctx {
+    int device;
+    hipStream_t stream;
+    hipError_t last_err;
+};
+
+enum PlanKind { PLAN_CSR = 1, PLAN_AXC = 2, PLAN_AXT = 3, PLAN_K1 = 4 };
+
+struct intspmv_plan {
+    intspmv_ctx *ctx;
+    int kind;
+    uint32_t nrows, nunits, nblocks, nsplit_rows, nlong_rows;
+    uint32_t umax, rmax, long_units;
+    ispv::BlockDesc *d_blocks;
+    double *d_partial;  // [nblocks]
+    uint32_t *d_cnt;    // [nblocks]
+    uint32_t chunkNum;  // K1
+    intspmv_axt_desc axt;
+};
+
+// internal helpers implemented in abi.hip
+int ispv_enter(intspmv_ctx *ctx);
+int ispv_launch_check(intspmv_ctx *ctx);
+int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
+                    uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_piece,
+                    intspmv_plan **out);

@@ -1,0 +1,118 @@
+// csr.hip -- CSR kernels (replacing cf_CSR, intSpmv_main.c:382-396).
+//
+// spmv_csr_ordered: one lane per row, fma in CSR order from +0.0 (the
+//   reference's sequential loop); bit-identical to it, not bandwidth-optimal
+//   for rows of very different lengths.
+// spmv_csr: plan blocks of <= 512 entries.  Phase 1 streams the block's
+//   values and columns with coalesced loads, gathers x[col] and stages both in
+//   LDS; phase 2 gives each row to one lane, which runs the same fma chain
+//   from LDS -- so every row of a normal block has exactly the ordered bits.
+//   Rows longer than 256 entries are reduced by a wave tree (and split over
+//   several waves beyond 512), within the FP64 tolerance.
+#include "common.hpp"
+#include "reduce.hpp"
+
+namespace ispv {
+
+constexpr uint32_t CSR_UMAX = 512;
+constexpr uint32_t CSR_RMAX = 256;
+constexpr uint32_t CSR_LONG = 256;
+
+__global__ __launch_bounds__(BLOCK_THREADS) void csr_ordered_kernel(
+    uint32_t nrows, const double *__restrict__ val, const uint32_t *__restrict__ col,
+    const uint32_t *__restrict__ row, const double *__restrict__ x, double *__restrict__ y)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    double acc = 0.0;
+    for (uint32_t k = row[r], e = row[r + 1]; k < e; k++) acc = fma(val[k], x[col[k]], acc);
+    y[r] = acc;
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
+    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const double *__restrict__ val,
+    const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+    const double *__restrict__ x, double *__restrict__ y, uint32_t nrows, double *partial,
+    uint32_t *cnt)
+{
+    __shared__ double s_v[WAVES][CSR_UMAX];
+    __shared__ double s_x[WAVES][CSR_UMAX];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    if (bid >= nblocks) return;
+    const BlockDesc d = blocks[bid];
+    const uint32_t nu = d.uend - d.ubeg;
+    constexpr int U = 8;
+    for (uint32_t base = 0; base < nu; base += WAVE * U) {
+        double v[U];
+        uint32_t c[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            uint32_t i = base + j * WAVE + lane;
+            i = i < nu ? i : nu - 1; // branch-free: re-read the last entry
+            v[j] = val[d.ubeg + i];
+            c[j] = col[d.ubeg + i];
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            uint32_t i = base + j * WAVE + lane;
+            if (i < nu) {
+                s_v[w][i] = v[j];
+                s_x[w][i] = x[c[j]];
+            }
+        }
+    }
+    wave_lds_sync();
+    if (d.k != 0) {
+        double acc = 0.0;
+        for (uint32_t i = lane; i < nu; i += WAVE) acc = fma(s_v[w][i], s_x[w][i], acc);
+        finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial, cnt);
+        return;
+    }
+    for (uint32_t r = d.rbeg + lane; r < d.rend; r += WAVE) {
+        uint32_t a = row[r] - d.ubeg, b = row[r + 1] - d.ubeg;
+        double acc = 0.0;
+        for (uint32_t i = a; i < b; i++) acc = fma(s_v[w][i], s_x[w][i], acc);
+        y[r] = acc;
+    }
+}
+
+} // namespace ispv
+
+using namespace ispv;
+
+extern "C" int intspmv_csr_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row_host,
+                                intspmv_plan **plan)
+{
+    if (!ctx || !plan || !row_host || row_host[0] != 0) return INTSPMV_ERR_ARG;
+    return ispv_plan_build(ctx, PLAN_CSR, nrows, row_host, CSR_UMAX, CSR_RMAX, CSR_LONG,
+                           CSR_UMAX, plan);
+}
+
+extern "C" int spmv_csr_ordered(intspmv_ctx *ctx, uint32_t nrows, const double *val,
+                                const uint32_t *col, const uint32_t *row, const double *x,
+                                double *y)
+{
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    if (nrows == 0) return INTSPMV_OK;
+    if (!row || !x || !y) return INTSPMV_ERR_ARG;
+    dim3 grid((nrows + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    csr_ordered_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(nrows, val, col, row, x, y);
+    return ispv_launch_check(ctx);
+}
+
+extern "C" int spmv_csr(intspmv_ctx *ctx, const intspmv_plan *plan, const double *val,
+                        const uint32_t *col, const uint32_t *row, const double *x, double *y)
+{
+    if (!plan || plan->kind != PLAN_CSR || plan->ctx != ctx) return INTSPMV_ERR_ARG;
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    if (plan->nblocks == 0) return INTSPMV_OK;
+    if (!row || !x || !y || (plan->nunits && (!val || !col))) return INTSPMV_ERR_ARG;
+    dim3 grid((plan->nblocks + WAVES - 1) / WAVES);
+    csr_block_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(
+        plan->d_blocks, plan->nblocks, val, col, row, x, y, plan->nrows, plan->d_partial,
+        plan->d_cnt);
+    return ispv_launch_check(ctx);
+}

@@ -1,0 +1,106 @@
+// reduce.hpp -- phase 2 of the block kernels: turning the per-unit sums a
+// wave staged in LDS into row results.
+//
+//   keyed:  each unit carries its row (AXT sec word); runs of equal keys are
+//           rows.  One lane per row adds the row's unit sums in unit order
+//           from +0.0, which is the order the reference's atomic
+//           accumulation takes when run serially (intSpmv_main.c:1400-1401,
+//           1469-1470).  Rows of the block with no unit are written as 0.
+//   rowptr: the format's row pointer gives each row's units (CSR row[],
+//           AXC brp[]); one lane per row, same left-to-right order.
+//   long:   the block is one row: lane-strided partial sums and a wave tree;
+//           a row split over several blocks meets in the plan's scratch.
+#pragma once
+#include "common.hpp"
+
+namespace ispv {
+
+template <typename AT>
+__device__ __forceinline__ AT wave_sum(AT v);
+template <>
+__device__ __forceinline__ double wave_sum<double>(double v) { return wave_sum_f64(v); }
+template <>
+__device__ __forceinline__ float wave_sum<float>(float v) { return wave_sum_f32(v); }
+
+// Compensated (Kahan-Babuska-Neumaier) running sum, used for FP32 rows.
+struct Neumaier {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float v)
+    {
+        float t = s + v;
+        if (fabsf(s) >= fabsf(v)) c += (s - t) + v;
+        else c += (v - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s + c; }
+};
+
+// Writes the long-row result (k == 1) or publishes the piece (k >= 2).
+template <typename AT>
+__device__ __forceinline__ void finish_long(AT total, const BlockDesc &d, uint32_t bid, AT *y,
+                                            uint32_t nrows, double *partial, uint32_t *cnt)
+{
+    if (d.k == 1) {
+        if (lane_id() == 0 && d.rbeg < nrows) y[d.rbeg] = total;
+    } else {
+        double tot;
+        if (split_publish((double)total, bid, d.b0, d.k, partial, cnt, &tot) && d.rbeg < nrows)
+            y[d.rbeg] = (AT)tot;
+    }
+}
+
+// Long row: all nu units of LDS array sv belong to row d.rbeg.
+template <typename AT>
+__device__ __forceinline__ void reduce_long(const AT *sv, uint32_t nu, const BlockDesc &d,
+                                            uint32_t bid, AT *y, uint32_t nrows,
+                                            double *partial, uint32_t *cnt)
+{
+    const int lane = lane_id();
+    AT acc = 0;
+    if constexpr (sizeof(AT) == 4) {
+        Neumaier k;
+        for (uint32_t i = lane; i < nu; i += WAVE) k.add(sv[i]);
+        acc = k.value();
+    } else {
+        for (uint32_t i = lane; i < nu; i += WAVE) acc = acc + sv[i];
+    }
+    finish_long<AT>(wave_sum<AT>(acc), d, bid, y, nrows, partial, cnt);
+}
+
+// Keyed segmentation.  sseg needs room for nu + 1 entries.
+template <typename AT>
+__device__ __forceinline__ void reduce_keyed(const AT *sv, const uint32_t *sk, uint16_t *sseg,
+                                             uint32_t nu, const BlockDesc &d, uint32_t bid, AT *y,
+                                             uint32_t nrows, double *partial, uint32_t *cnt)
+{
+    const int lane = lane_id();
+    if (d.k != 0) {
+        reduce_long<AT>(sv, nu, d, bid, y, nrows, partial, cnt);
+        return;
+    }
+    uint32_t nseg = 0;
+    for (uint32_t b = 0; b < nu; b += WAVE) {
+        uint32_t p = b + lane;
+        bool head = p < nu && (p == 0 || sk[p] != sk[p - 1]);
+        uint64_t m = __ballot(head);
+        if (head) sseg[nseg + __popcll(m & lanemask_lt())] = (uint16_t)p;
+        nseg += __popcll(m);
+    }
+    if (lane == 0) sseg[nseg] = (uint16_t)nu;
+    wave_lds_sync();
+    for (uint32_t s = lane; s < nseg; s += WAVE) {
+        uint32_t a = sseg[s], b = sseg[s + 1];
+        AT acc = 0;
+        for (uint32_t i = a; i < b; i++) acc = acc + sv[i];
+        uint32_t r = sk[a];
+        if (r < nrows) y[r] = acc;
+        uint32_t lo = (s == 0) ? d.rbeg : sk[sseg[s - 1]] + 1;
+        for (uint32_t z = lo; z < r && z < nrows; z++) y[z] = 0;
+        if (s == nseg - 1)
+            for (uint32_t z = r + 1; z < d.rend && z < nrows; z++) y[z] = 0;
+    }
+    if (nseg == 0)
+        for (uint32_t z = d.rbeg + lane; z < d.rend && z < nrows; z += WAVE) y[z] = 0;
+}
+
+} // namespace ispv

@@ -1,0 +1,302 @@
+"""Device side: ctypes bindings of libintspmv.so (include/intspmv.h).
+
+Device memory and streams come from PyTorch (plumbing only): arrays are
+uploaded as torch tensors on ``cuda:N`` and their raw pointers are passed to
+the C ABI, which launches the hand-written gfx950 kernels on the context's
+stream (torch's current stream unless set otherwise).
+
+The classes mirror the reference's formats and kernels (intSpmv_main.c):
+
+  DeviceCSR.cf_csr / .spmv         cf_CSR              (382-396)
+  DeviceAXC.int_axc                int_axc             (711-732)
+  DeviceK1.int_k1 / .int_k1p       int_k1, int_k1p     (943-1066)
+  DeviceAXT.int_axt                int_axt_noc_h1 / int_axt_noc (1385-1474)
+
+Every method overwrites y (y = A x).  There is no CPU fallback: without the
+built library or a GPU the constructors raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._loader import load_library
+
+F64, F32, BF16 = 0, 1, 2
+NOC, COM = 0, 1
+INTERLEAVED, PLANAR = 0, 1
+
+_vp = C.c_void_p
+_u32 = C.c_uint32
+
+
+class IntSpmvError(RuntimeError):
+    def __init__(self, status: int, what: str, detail: str = ""):
+        msg = f"{what}: {lib().intspmv_strerror(status).decode()} ({status})"
+        if detail:
+            msg += f" [{detail}]"
+        super().__init__(msg)
+        self.status = status
+
+
+class AxtDesc(C.Structure):
+    _fields_ = [("nrows", _u32), ("nnz", _u32), ("tileHW", _u32), ("tileH", _u32),
+                ("tileN", _u32), ("lenAX", _u32), ("lenSEC", _u32), ("colN", _u32),
+                ("bs", _u32), ("log", _u32), ("mode", C.c_int32), ("val_type", C.c_int32),
+                ("x_type", C.c_int32), ("layout", C.c_int32)]
+
+
+# (name, restype, argtypes) of every exported symbol of include/intspmv.h
+SIGNATURES = [
+    ("intspmv_create", C.c_int, [C.POINTER(_vp), C.c_int]),
+    ("intspmv_destroy", C.c_int, [_vp]),
+    ("intspmv_set_stream", C.c_int, [_vp, _vp]),
+    ("intspmv_sync", C.c_int, [_vp]),
+    ("intspmv_strerror", C.c_char_p, [C.c_int]),
+    ("intspmv_last_hip_error", C.c_char_p, [_vp]),
+    ("intspmv_device_count", C.c_int, [C.POINTER(C.c_int)]),
+    ("intspmv_malloc", C.c_int, [_vp, C.POINTER(_vp), C.c_size_t]),
+    ("intspmv_free", C.c_int, [_vp, _vp]),
+    ("intspmv_memcpy_htod", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    ("intspmv_memcpy_dtoh", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    ("intspmv_csr_plan", C.c_int, [_vp, _u32, _vp, C.POINTER(_vp)]),
+    ("spmv_csr_ordered", C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+    ("spmv_csr", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("intspmv_axc_plan", C.c_int, [_vp, _u32, _vp, C.POINTER(_vp)]),
+    ("spmv_axc", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("intspmv_k1_plan", C.c_int, [_vp, _u32, _u32, _vp, C.POINTER(_vp)]),
+    ("spmv_k1", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("spmv_k1_order", C.c_int, [_vp, _u32, _vp, _vp, _vp]),
+    ("intspmv_axt_plan", C.c_int, [_vp, C.POINTER(AxtDesc), _vp, C.POINTER(_vp)]),
+    ("spmv_axt", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("intspmv_plan_destroy", C.c_int, [_vp]),
+    ("intspmv_plan_info", C.c_int, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = load_library("libintspmv.so")
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    lib().intspmv_device_count(C.byref(n))
+    return n.value
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _np_ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _dptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class Engine:
+    """Owns an intspmv context on one GPU; launches go to torch's current stream
+    of that device unless :meth:`set_stream` is called."""
+
+    def __init__(self, device: int = 0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("int_spmv_amd needs a HIP GPU; none is visible")
+        self.device = device
+        self.torch_device = torch.device("cuda", device)
+        h = _vp()
+        self._check(lib().intspmv_create(C.byref(h), device), "intspmv_create", ctx=False)
+        self.h = h
+        self.set_stream(torch.cuda.current_stream(self.torch_device))
+
+    def _check(self, st: int, what: str, ctx: bool = True) -> None:
+        if st != 0:
+            detail = lib().intspmv_last_hip_error(self.h).decode() if ctx else ""
+            raise IntSpmvError(st, what, detail)
+
+    def set_stream(self, stream) -> None:
+        self.stream = stream
+        self._check(lib().intspmv_set_stream(self.h, _vp(stream.cuda_stream)), "set_stream")
+
+    def sync(self) -> None:
+        self._check(lib().intspmv_sync(self.h), "sync")
+
+    def upload(self, a: np.ndarray):
+        torch = _torch()
+        a = np.ascontiguousarray(a)
+        if a.dtype == np.uint32:  # torch has no uint32 arithmetic; keep the bits
+            t = torch.from_numpy(a.view(np.int32)).to(self.torch_device)
+        elif a.dtype == np.uint16:
+            t = torch.from_numpy(a.view(np.int16)).to(self.torch_device)
+        else:
+            t = torch.from_numpy(a).to(self.torch_device)
+        return t
+
+    def empty(self, n: int, dtype=np.float64):
+        torch = _torch()
+        tdt = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+               np.dtype(np.uint32): torch.int32}[np.dtype(dtype)]
+        return torch.empty(max(n, 1), dtype=tdt, device=self.torch_device)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.intspmv_destroy(self.h)
+            self.h = None
+
+
+class _Plan:
+    def __init__(self, eng: Engine, h: _vp):
+        self.eng, self.h = eng, h
+
+    def info(self) -> dict:
+        a, b, c = _u32(), _u32(), _u32()
+        lib().intspmv_plan_info(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return dict(nblocks=a.value, nsplit_rows=b.value, nlong_rows=c.value)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.intspmv_plan_destroy(self.h)
+            self.h = None
+
+
+class DeviceCSR:
+    def __init__(self, eng: Engine, m):
+        self.eng, self.nrows, self.nnz = eng, m.nrows, m.nnz
+        self.val = eng.upload(m.val)
+        self.col = eng.upload(m.col)
+        self.row = eng.upload(m.row)
+        h = _vp()
+        row = np.ascontiguousarray(m.row)
+        eng._check(lib().intspmv_csr_plan(eng.h, m.nrows, _np_ptr(row), C.byref(h)), "csr_plan")
+        self.plan = _Plan(eng, h)
+
+    def cf_csr(self, x, y) -> None:
+        """Ordered product, same bits as cf_CSR (intSpmv_main.c:382-396)."""
+        self.eng._check(lib().spmv_csr_ordered(self.eng.h, self.nrows, _dptr(self.val),
+                                               _dptr(self.col), _dptr(self.row), _dptr(x),
+                                               _dptr(y)), "spmv_csr_ordered")
+
+    def spmv(self, x, y) -> None:
+        self.eng._check(lib().spmv_csr(self.eng.h, self.plan.h, _dptr(self.val), _dptr(self.col),
+                                       _dptr(self.row), _dptr(x), _dptr(y)), "spmv_csr")
+
+
+class DeviceAXC:
+    def __init__(self, eng: Engine, a):
+        self.eng, self.nrows = eng, a.nrows
+        self.ax = eng.upload(a.ax)
+        self.brp = eng.upload(a.brp)
+        h = _vp()
+        brp = np.ascontiguousarray(a.brp)
+        eng._check(lib().intspmv_axc_plan(eng.h, a.nrows, _np_ptr(brp), C.byref(h)), "axc_plan")
+        self.plan = _Plan(eng, h)
+
+    def int_axc(self, y) -> None:
+        self.eng._check(lib().spmv_axc(self.eng.h, self.plan.h, _dptr(self.ax), _dptr(self.brp),
+                                       _dptr(y)), "spmv_axc")
+
+
+class DeviceK1:
+    def __init__(self, eng: Engine, k):
+        self.eng, self.nrows, self.chunkNum = eng, k.nrows, k.chunkNum
+        self.val = eng.upload(k.val)
+        self.col = eng.upload(k.col)
+        self.chp = eng.upload(k.chp)
+        self.nmc = eng.upload(k.nmc)
+        self.permi = eng.upload(k.permi)
+        h = _vp()
+        nmc = np.ascontiguousarray(k.nmc)
+        eng._check(lib().intspmv_k1_plan(eng.h, k.nrows, k.chunkNum, _np_ptr(nmc), C.byref(h)),
+                   "k1_plan")
+        self.plan = _Plan(eng, h)
+
+    def int_k1(self, x, res) -> None:
+        """Permuted result res[chunkNum*8] (int_k1, intSpmv_main.c:943-965)."""
+        self.eng._check(lib().spmv_k1(self.eng.h, self.plan.h, _dptr(self.val), _dptr(self.col),
+                                      _dptr(self.chp), _dptr(self.nmc), None, _dptr(x),
+                                      _dptr(res)), "spmv_k1")
+
+    def order(self, res, y) -> None:
+        """orderArrayK1 (intSpmv_main.c:969-976)."""
+        self.eng._check(lib().spmv_k1_order(self.eng.h, self.nrows, _dptr(self.permi), _dptr(res),
+                                            _dptr(y)), "spmv_k1_order")
+
+    def int_k1p(self, x, y) -> None:
+        """Scatter variant (int_k1p, intSpmv_main.c:1026-1066)."""
+        self.eng._check(lib().spmv_k1(self.eng.h, self.plan.h, _dptr(self.val), _dptr(self.col),
+                                      _dptr(self.chp), _dptr(self.nmc), _dptr(self.permi),
+                                      _dptr(x), _dptr(y)), "spmv_k1p")
+
+
+def bf16_bits(a: np.ndarray) -> np.ndarray:
+    """Round-to-nearest-even FP32 -> BF16 bit patterns (uint16)."""
+    f = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((f + 0x7FFF + ((f >> 16) & 1)) >> 16).astype(np.uint16)
+    return r
+
+
+def bf16_to_f32(b: np.ndarray) -> np.ndarray:
+    return (b.astype(np.uint32) << 16).view(np.float32)
+
+
+def axt_planes(a) -> tuple[np.ndarray, np.ndarray]:
+    """Splits the interleaved ax of an AXT host object into value and x planes
+    in slot order (slot = T*th*8 + R*8 + C)."""
+    ax = a.ax.reshape(-1, 2, 8)
+    return np.ascontiguousarray(ax[:, 0, :]).reshape(-1), np.ascontiguousarray(ax[:, 1, :]).reshape(-1)
+
+
+class DeviceAXT:
+    """AXT NOC on the device.  ``val_type`` / ``x_type`` select the stored
+    precision: (F64, F64) is the reference layout (interleaved ax); (F32, F32)
+    interleaved FP32; (BF16, F32) uses planar value / x arrays."""
+
+    def __init__(self, eng: Engine, a, val_type: int = F64, x_type: int = F64,
+                 layout: int | None = None):
+        if layout is None:
+            layout = PLANAR if val_type == BF16 else INTERLEAVED
+        self.eng, self.nrows = eng, a.nrows
+        self.val_type, self.x_type, self.layout = val_type, x_type, layout
+        d = AxtDesc(nrows=a.nrows, nnz=a.nnz, tileHW=a.tileHW, tileH=a.tileH, tileN=a.tileN,
+                    lenAX=a.lenAX, lenSEC=a.lenSEC, colN=a.colN, bs=a.bs, log=a.log,
+                    mode=NOC if a.mode == "NOC" else COM, val_type=val_type, x_type=x_type,
+                    layout=layout)
+        self.desc = d
+        self.xplane = None
+        if layout == INTERLEAVED:
+            if val_type == F64 and x_type == F64:
+                self.ax = eng.upload(a.ax)
+            elif val_type == F32 and x_type == F32:
+                self.ax = eng.upload(a.ax.astype(np.float32))
+            else:
+                raise ValueError("interleaved layout needs equal value and x types")
+        else:
+            vp, xp = axt_planes(a)
+            self.ax = eng.upload({F64: vp, F32: vp.astype(np.float32),
+                                  BF16: bf16_bits(vp)}[val_type])
+            self.xplane = eng.upload({F64: xp, F32: xp.astype(np.float32)}[x_type])
+        self.sec = eng.upload(a.sec)
+        h = _vp()
+        con = np.ascontiguousarray(a.con)
+        eng._check(lib().intspmv_axt_plan(eng.h, C.byref(d), _np_ptr(con), C.byref(h)),
+                   "axt_plan")
+        self.plan = _Plan(eng, h)
+        self.ydtype = np.float64 if val_type == F64 else np.float32
+
+    def int_axt(self, y) -> None:
+        self.eng._check(lib().spmv_axt(self.eng.h, self.plan.h, _dptr(self.ax), _dptr(self.xplane),
+                                       _dptr(self.sec), _dptr(y)), "spmv_axt")
