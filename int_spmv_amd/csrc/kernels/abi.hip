@@ -161,6 +161,58 @@ extern "C" int intspmv_memcpy_dtoh(intspmv_ctx *ctx, void *dst, const void *src,
     return copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost);
 }
 
+// Split-row fix-up: one thread per split row adds the row's piece partials
+// in piece order (deterministic) and stores the result.
+template <typename YT>
+__global__ void split_fixup_kernel(const ispv::SplitDesc *__restrict__ splits, uint32_t nsplit,
+                                   const double *__restrict__ partial, YT *__restrict__ y,
+                                   uint32_t nrows)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsplit) return;
+    const ispv::SplitDesc s = splits[i];
+    double t = 0.0;
+    for (uint32_t j = 0; j < s.k; j++) t = t + partial[s.b0 + j];
+    if (s.row < nrows) y[s.row] = (YT)t;
+}
+
+int ispv_split_fixup(const intspmv_plan *p, void *y, int y_is_f32)
+{
+    if (p->nsplit_rows == 0) return INTSPMV_OK;
+    dim3 grid((p->nsplit_rows + 255) / 256);
+    if (y_is_f32)
+        split_fixup_kernel<float><<<grid, 256, 0, p->ctx->stream>>>(
+            p->d_splits, p->nsplit_rows, p->d_partial, (float *)y, p->nrows);
+    else
+        split_fixup_kernel<double><<<grid, 256, 0, p->ctx->stream>>>(
+            p->d_splits, p->nsplit_rows, p->d_partial, (double *)y, p->nrows);
+    return ispv_launch_check(p->ctx);
+}
+
+// Allocates and fills the device side of a plan whose host fields are set.
+int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size_t block_bytes,
+                     size_t nblocks, size_t partial_per_block, const ispv::SplitDesc *splits,
+                     size_t nsplit)
+{
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    size_t nb = nblocks ? nblocks : 1, ns = nsplit ? nsplit : 1;
+    hipError_t e = hipMalloc(&p->d_blocks, nb * block_bytes);
+    if (e == hipSuccess) e = hipMalloc(&p->d_partial, nb * partial_per_block * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&p->d_splits, ns * sizeof(ispv::SplitDesc));
+    if (e == hipSuccess && nblocks)
+        e = hipMemcpy(p->d_blocks, blocks, nblocks * block_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nsplit)
+        e = hipMemcpy(p->d_splits, splits, nsplit * sizeof(ispv::SplitDesc),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        return INTSPMV_ERR_NOMEM;
+    }
+    return INTSPMV_OK;
+}
+
 // Packs rows into blocks.  uptr[r]..uptr[r+1] are row r's units (monotone,
 // n+1 entries).  Rows with more than long_units units become long blocks,
 // split into ceil(len / long_piece) near-equal pieces when longer than
@@ -176,7 +228,8 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
     for (uint32_t r = 0; r < nrows; r++)
         if (uptr[r + 1] < uptr[r]) return INTSPMV_ERR_ARG;
     std::vector<ispv::BlockDesc> blocks;
-    uint32_t nsplit = 0, nlong = 0;
+    std::vector<ispv::SplitDesc> splits;
+    uint32_t nlong = 0;
     uint32_t r = 0;
     while (r < nrows) {
         uint32_t len = uptr[r + 1] - uptr[r];
@@ -188,12 +241,12 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
                 nlong++;
             } else {
                 uint64_t per = (len + (uint64_t)np - 1) / np;
-                for (uint32_t p = 0; p < np; p++) {
-                    uint64_t a = uptr[r] + p * per, b = a + per;
+                for (uint32_t q = 0; q < np; q++) {
+                    uint64_t a = uptr[r] + q * per, b = a + per;
                     if (b > uptr[r + 1]) b = uptr[r + 1];
                     blocks.push_back({(uint32_t)a, (uint32_t)b, r, r + 1, b0, np, 0, 0});
                 }
-                nsplit++;
+                splits.push_back({r, b0, np, 0});
             }
             r++;
             continue;
@@ -214,29 +267,16 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
     p->nrows = nrows;
     p->nunits = nrows ? uptr[nrows] : 0;
     p->nblocks = (uint32_t)blocks.size();
-    p->nsplit_rows = nsplit;
+    p->nsplit_rows = (uint32_t)splits.size();
     p->nlong_rows = nlong;
     p->umax = umax;
     p->rmax = rmax;
     p->long_units = long_units;
-    size_t nb = blocks.size() ? blocks.size() : 1;
-    int st = ispv_enter(ctx);
+    int st = ispv_plan_upload(ctx, p, blocks.data(), sizeof(ispv::BlockDesc), blocks.size(), 1,
+                              splits.data(), splits.size());
     if (st) {
-        free(p);
-        return st;
-    }
-    hipError_t e = hipMalloc(&p->d_blocks, nb * sizeof(ispv::BlockDesc));
-    if (e == hipSuccess) e = hipMalloc(&p->d_partial, nb * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&p->d_cnt, nb * sizeof(uint32_t));
-    if (e == hipSuccess && !blocks.empty())
-        e = hipMemcpy(p->d_blocks, blocks.data(), blocks.size() * sizeof(ispv::BlockDesc),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(p->d_cnt, 0, nb * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
-        ctx->last_err = e;
         intspmv_plan_destroy(p);
-        return INTSPMV_ERR_NOMEM;
+        return st;
     }
     *out = p;
     return INTSPMV_OK;
@@ -248,7 +288,7 @@ extern "C" int intspmv_plan_destroy(intspmv_plan *p)
     if (p->ctx) ispv_enter(p->ctx);
     if (p->d_blocks) (void)hipFree(p->d_blocks);
     if (p->d_partial) (void)hipFree(p->d_partial);
-    if (p->d_cnt) (void)hipFree(p->d_cnt);
+    if (p->d_splits) (void)hipFree(p->d_splits);
     free(p);
     return INTSPMV_OK;
 }

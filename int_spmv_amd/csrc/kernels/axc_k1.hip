@@ -23,12 +23,11 @@ namespace ispv {
 constexpr uint32_t AXC_UMAX = 256;
 constexpr uint32_t AXC_RMAX = 256;
 constexpr uint32_t AXC_LONG = 64;
-constexpr uint32_t K1_LONG = 512; // steps per K1 work item
+constexpr uint32_t K1_LONG = 256; // steps per K1 work item
 
 __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nblocks, const double *__restrict__ ax,
-    const uint32_t *__restrict__ brp, double *__restrict__ y, uint32_t nrows, double *partial,
-    uint32_t *cnt)
+    const uint32_t *__restrict__ brp, double *__restrict__ y, uint32_t nrows, double *partial)
 {
     __shared__ double s_val[WAVES][AXC_UMAX];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -60,7 +59,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
     }
     wave_lds_sync();
     if (d.k != 0) {
-        reduce_long<double>(s_val[w], nu, d, bid, y, nrows, partial, cnt);
+        reduce_long<double>(s_val[w], nu, d, bid, y, nrows, partial);
         return;
     }
     for (uint32_t r = d.rbeg + lane; r < d.rend; r += WAVE) {
@@ -80,7 +79,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k1_kernel(
     const K1Item *__restrict__ items, uint32_t nitems, const double *__restrict__ val,
     const uint32_t *__restrict__ col, const uint32_t *__restrict__ chp,
     const uint32_t *__restrict__ permi, const double *__restrict__ x, double *__restrict__ y,
-    uint32_t nrows, double *partial, uint32_t *cnt)
+    uint32_t nrows, double *partial)
 {
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
     const uint32_t wid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
@@ -108,33 +107,31 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k1_kernel(
         }
         for (; k < d.kend; k++) acc = fma(val[base + 8 * k], x[col[base + 8 * k]], acc);
     }
-    // write or combine
     const uint32_t s = d.c * 8 + l;
-    if (have && d.k <= 1) {
+    if (!have) return;
+    if (d.k <= 1) {
         if (!permi) y[s] = acc;
         else if (s < nrows) y[permi[s]] = acc;
+    } else {
+        partial[(size_t)it * 8 + l] = acc; // combined by k1_fixup_kernel
     }
-    // split pieces only: publish, count, and let the last piece combine
-    if (!__any(have && d.k > 1)) return;
-    if (have && d.k > 1) {
-        __hip_atomic_store(&partial[(size_t)it * 8 + l], acc, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    uint32_t old = 0;
-    if (have && d.k > 1 && l == 0)
-        old = __hip_atomic_fetch_add(&cnt[d.b0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    old = __shfl(old, lane & ~7, WAVE);
-    if (have && d.k > 1 && old == d.k - 1) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        double tot = 0.0;
-        for (uint32_t j = 0; j < d.k; j++)
-            tot = tot + __hip_atomic_load(&partial[(size_t)(d.b0 + j) * 8 + l], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-        if (!permi) y[s] = tot;
-        else if (s < nrows) y[permi[s]] = tot;
-        if (l == 0) __hip_atomic_store(&cnt[d.b0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+}
+
+// Split K1 chunks: 8 threads per chunk add the pieces' lane partials in
+// piece order and store like the main kernel.
+__global__ void k1_fixup_kernel(const SplitDesc *__restrict__ splits, uint32_t nsplit,
+                                const double *__restrict__ partial,
+                                const uint32_t *__restrict__ permi, double *__restrict__ y,
+                                uint32_t nrows)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, i = t / 8, l = t % 8;
+    if (i >= nsplit) return;
+    const SplitDesc sp = splits[i];
+    double tot = 0.0;
+    for (uint32_t j = 0; j < sp.k; j++) tot = tot + partial[(size_t)(sp.b0 + j) * 8 + l];
+    uint32_t s = sp.row * 8 + l;
+    if (!permi) y[s] = tot;
+    else if (s < nrows) y[permi[s]] = tot;
 }
 
 __global__ void k1_order_kernel(uint32_t nrows, const uint32_t *__restrict__ permi,
@@ -177,9 +174,10 @@ extern "C" int spmv_axc(intspmv_ctx *ctx, const intspmv_plan *plan, const double
     if (!brp || !y || (plan->nunits && !ax)) return INTSPMV_ERR_ARG;
     dim3 grid((plan->nblocks + WAVES - 1) / WAVES);
     axc_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(plan->d_blocks, plan->nblocks, ax, brp,
-                                                        y, plan->nrows, plan->d_partial,
-                                                        plan->d_cnt);
-    return ispv_launch_check(ctx);
+                                                        y, plan->nrows, plan->d_partial);
+    st = ispv_launch_check(ctx);
+    if (st) return st;
+    return ispv_split_fixup(plan, y, 0);
 }
 
 extern "C" int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkNum,
@@ -189,7 +187,7 @@ extern "C" int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkN
     if (chunkNum != (nrows + 7) / 8) return INTSPMV_ERR_ARG;
     *plan = nullptr;
     std::vector<K1Item> items;
-    uint32_t nsplit = 0;
+    std::vector<SplitDesc> splits;
     for (uint32_t c = 0; c < chunkNum; c++) {
         uint32_t n = nmc_host[c];
         if (n <= K1_LONG) {
@@ -198,11 +196,11 @@ extern "C" int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkN
         }
         uint32_t np = (n + K1_LONG - 1) / K1_LONG, per = (n + np - 1) / np;
         uint32_t b0 = (uint32_t)items.size();
-        for (uint32_t p = 0; p < np; p++) {
-            uint32_t a = p * per, b = a + per > n ? n : a + per;
+        for (uint32_t q = 0; q < np; q++) {
+            uint32_t a = q * per, b = a + per > n ? n : a + per;
             items.push_back({c, a, b, b0, np, 0, 0, 0});
         }
-        nsplit++;
+        splits.push_back({c, b0, np, 0});
     }
     intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));
     if (!p) return INTSPMV_ERR_NOMEM;
@@ -211,25 +209,12 @@ extern "C" int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkN
     p->nrows = nrows;
     p->chunkNum = chunkNum;
     p->nblocks = (uint32_t)items.size();
-    p->nsplit_rows = nsplit;
-    int st = ispv_enter(ctx);
+    p->nsplit_rows = (uint32_t)splits.size();
+    int st = ispv_plan_upload(ctx, p, items.data(), sizeof(K1Item), items.size(), 8,
+                              splits.data(), splits.size());
     if (st) {
-        free(p);
-        return st;
-    }
-    size_t ni = items.empty() ? 1 : items.size();
-    hipError_t e = hipMalloc(&p->d_blocks, ni * sizeof(K1Item));
-    if (e == hipSuccess) e = hipMalloc(&p->d_partial, ni * 8 * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&p->d_cnt, ni * sizeof(uint32_t));
-    if (e == hipSuccess && !items.empty())
-        e = hipMemcpy(p->d_blocks, items.data(), items.size() * sizeof(K1Item),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(p->d_cnt, 0, ni * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
-        ctx->last_err = e;
         intspmv_plan_destroy(p);
-        return INTSPMV_ERR_NOMEM;
+        return st;
     }
     *plan = p;
     return INTSPMV_OK;
@@ -249,7 +234,11 @@ extern "C" int spmv_k1(intspmv_ctx *ctx, const intspmv_plan *plan, const double 
     dim3 grid((waves + WAVES - 1) / WAVES);
     k1_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(
         (const K1Item *)plan->d_blocks, plan->nblocks, val, col, chp, permi, x, y, plan->nrows,
-        plan->d_partial, plan->d_cnt);
+        plan->d_partial);
+    st = ispv_launch_check(ctx);
+    if (st || plan->nsplit_rows == 0) return st;
+    k1_fixup_kernel<<<(plan->nsplit_rows * 8 + 255) / 256, 256, 0, ctx->stream>>>(
+        plan->d_splits, plan->nsplit_rows, plan->d_partial, permi, y, plan->nrows);
     return ispv_launch_check(ctx);
 }
 

@@ -95,7 +95,7 @@ template <typename VT, typename XT, typename AT, bool PLANAR>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_h1_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ av,
     const XT *__restrict__ axv, const uint32_t *__restrict__ sec, AT *__restrict__ y,
-    uint32_t nrows, double *partial, uint32_t *cnt)
+    uint32_t nrows, double *partial)
 {
     __shared__ AT s_val[WAVES][AXT_UMAX];
     __shared__ uint32_t s_key[WAVES][AXT_UMAX];
@@ -138,14 +138,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_h1_kernel(
         }
     }
     wave_lds_sync();
-    reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial, cnt);
+    reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
 }
 
 template <typename VT, typename XT, typename AT, bool PLANAR, int TH>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ av,
     const XT *__restrict__ axv, const uint32_t *__restrict__ sec, AT *__restrict__ y,
-    uint32_t nrows, double *partial, uint32_t *cnt, uint32_t th_rt)
+    uint32_t nrows, double *partial, uint32_t th_rt)
 {
     __shared__ AT s_val[WAVES][AXT_UMAX];
     __shared__ uint32_t s_key[WAVES][AXT_UMAX];
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
         }
     }
     wave_lds_sync();
-    reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial, cnt);
+    reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
 }
 
 template <typename VT, typename XT, typename AT, bool PLANAR>
@@ -222,12 +222,12 @@ static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, con
     auto yy = (AT *)y;
     if (th == 1) {
         axt_h1_kernel<VT, XT, AT, PLANAR><<<grid, blk, 0, s>>>(
-            p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, p->d_cnt);
+            p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial);
     } else {
 #define ISPV_TH(N)                                                                            \
     case N:                                                                                   \
         axt_noc_kernel<VT, XT, AT, PLANAR, N><<<grid, blk, 0, s>>>(                           \
-            p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, p->d_cnt, th);            \
+            p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, th);            \
         break;
         switch (th) {
             ISPV_TH(2)
@@ -236,11 +236,13 @@ static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, con
             ISPV_TH(16)
         default:
             axt_noc_kernel<VT, XT, AT, PLANAR, 0><<<grid, blk, 0, s>>>(
-                p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, p->d_cnt, th);
+                p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, th);
         }
 #undef ISPV_TH
     }
-    return ispv_launch_check(p->ctx);
+    int st = ispv_launch_check(p->ctx);
+    if (st) return st;
+    return ispv_split_fixup(p, y, sizeof(AT) == 4);
 }
 
 } // namespace ispv

@@ -25,8 +25,8 @@ constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, workgroups dispatched rou
 //   k == 0 : normal block, whole rows, each summed by one lane in unit order;
 //   k == 1 : one long row, reduced by a wave tree, written directly;
 //   k >= 2 : piece of a long row split over k consecutive blocks b0..b0+k-1;
-//            partial sums meet in the plan's scratch, the last wave to arrive
-//            adds them in piece order.
+//            each piece leaves its partial sum in the plan's scratch and the
+//            split fix-up kernel adds them in piece order.
 struct BlockDesc {
     uint32_t ubeg, uend, rbeg, rend;
     uint32_t b0, k, pad0, pad1;
@@ -92,31 +92,14 @@ __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) -
 // ---- bf16 ----------------------------------------------------------------
 __device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// ---- split-row combination ----------------------------------------------
-// Publishes this block's partial and, when it is the last of the k pieces of
-// its row to arrive, returns true with the ordered sum of all k partials in
-// *total.  No wave ever waits for another: the last arriver does the sum.
-// Counters reset themselves, so the scratch needs no clearing between calls.
-__device__ __forceinline__ bool split_publish(double part, uint32_t blk, uint32_t b0, uint32_t k,
-                                             double *partial, uint32_t *cnt, double *total)
-{
-    bool last = false;
-    if (lane_id() == 0) {
-        __hip_atomic_store(&partial[blk], part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t old = __hip_atomic_fetch_add(&cnt[b0], 1u, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-        if (old == k - 1) {
-            double s = 0.0;
-            for (uint32_t j = 0; j < k; j++)
-                s = s + __hip_atomic_load(&partial[b0 + j], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&cnt[b0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            *total = s;
-            last = true;
-        }
-    }
-    return last;
-}
+// ---- split rows ----------------------------------------------------------
+// A row longer than one block is cut into k pieces b0..b0+k-1; each piece
+// writes its partial sum to partial[block] and a second, small kernel adds
+// the k partials in piece order once the main kernel has finished (the kernel
+// boundary orders the writes; no device-wide fences in the streaming kernel).
+struct SplitDesc {
+    uint32_t row, b0, k, pad;
+};
 
 } // namespace ispv
 
@@ -135,8 +118,8 @@ struct intspmv_plan {
     uint32_t nrows, nunits, nblocks, nsplit_rows, nlong_rows;
     uint32_t umax, rmax, long_units;
     ispv::BlockDesc *d_blocks;
-    double *d_partial;  // [nblocks]
-    uint32_t *d_cnt;    // [nblocks]
+    double *d_partial;          // [nblocks] (K1: [nitems * 8])
+    ispv::SplitDesc *d_splits;  // [nsplit_rows]
     uint32_t chunkNum;  // K1
     intspmv_axt_desc axt;
 };
@@ -144,6 +127,10 @@ struct intspmv_plan {
 // internal helpers implemented in abi.hip
 int ispv_enter(intspmv_ctx *ctx);
 int ispv_launch_check(intspmv_ctx *ctx);
+int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size_t block_bytes,
+                     size_t nblocks, size_t partial_per_block, const ispv::SplitDesc *splits,
+                     size_t nsplit);
+int ispv_split_fixup(const intspmv_plan *p, void *y, int y_is_f32);
 int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
                     uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_piece,
                     intspmv_plan **out);

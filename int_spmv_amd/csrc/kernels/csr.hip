@@ -32,8 +32,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_ordered_kernel(
 __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nblocks, const double *__restrict__ val,
     const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
-    const double *__restrict__ x, double *__restrict__ y, uint32_t nrows, double *partial,
-    uint32_t *cnt)
+    const double *__restrict__ x, double *__restrict__ y, uint32_t nrows, double *partial)
 {
     __shared__ double s_v[WAVES][CSR_UMAX];
     __shared__ double s_x[WAVES][CSR_UMAX];
@@ -66,7 +65,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
     if (d.k != 0) {
         double acc = 0.0;
         for (uint32_t i = lane; i < nu; i += WAVE) acc = fma(s_v[w][i], s_x[w][i], acc);
-        finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial, cnt);
+        finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial);
         return;
     }
     for (uint32_t r = d.rbeg + lane; r < d.rend; r += WAVE) {
@@ -112,7 +111,8 @@ extern "C" int spmv_csr(intspmv_ctx *ctx, const intspmv_plan *plan, const double
     if (!row || !x || !y || (plan->nunits && (!val || !col))) return INTSPMV_ERR_ARG;
     dim3 grid((plan->nblocks + WAVES - 1) / WAVES);
     csr_block_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(
-        plan->d_blocks, plan->nblocks, val, col, row, x, y, plan->nrows, plan->d_partial,
-        plan->d_cnt);
-    return ispv_launch_check(ctx);
+        plan->d_blocks, plan->nblocks, val, col, row, x, y, plan->nrows, plan->d_partial);
+    st = ispv_launch_check(ctx);
+    if (st) return st;
+    return ispv_split_fixup(plan, y, 0);
 }

@@ -35,17 +35,16 @@ struct Neumaier {
     __device__ __forceinline__ float value() const { return s + c; }
 };
 
-// Writes the long-row result (k == 1) or publishes the piece (k >= 2).
+// Writes the long-row result (k == 1) or this piece's partial (k >= 2).
 template <typename AT>
 __device__ __forceinline__ void finish_long(AT total, const BlockDesc &d, uint32_t bid, AT *y,
-                                            uint32_t nrows, double *partial, uint32_t *cnt)
+                                            uint32_t nrows, double *partial)
 {
+    if (lane_id() != 0) return;
     if (d.k == 1) {
-        if (lane_id() == 0 && d.rbeg < nrows) y[d.rbeg] = total;
+        if (d.rbeg < nrows) y[d.rbeg] = total;
     } else {
-        double tot;
-        if (split_publish((double)total, bid, d.b0, d.k, partial, cnt, &tot) && d.rbeg < nrows)
-            y[d.rbeg] = (AT)tot;
+        partial[bid] = (double)total;
     }
 }
 
@@ -53,7 +52,7 @@ __device__ __forceinline__ void finish_long(AT total, const BlockDesc &d, uint32
 template <typename AT>
 __device__ __forceinline__ void reduce_long(const AT *sv, uint32_t nu, const BlockDesc &d,
                                             uint32_t bid, AT *y, uint32_t nrows,
-                                            double *partial, uint32_t *cnt)
+                                            double *partial)
 {
     const int lane = lane_id();
     AT acc = 0;
@@ -64,18 +63,18 @@ __device__ __forceinline__ void reduce_long(const AT *sv, uint32_t nu, const Blo
     } else {
         for (uint32_t i = lane; i < nu; i += WAVE) acc = acc + sv[i];
     }
-    finish_long<AT>(wave_sum<AT>(acc), d, bid, y, nrows, partial, cnt);
+    finish_long<AT>(wave_sum<AT>(acc), d, bid, y, nrows, partial);
 }
 
 // Keyed segmentation.  sseg needs room for nu + 1 entries.
 template <typename AT>
 __device__ __forceinline__ void reduce_keyed(const AT *sv, const uint32_t *sk, uint16_t *sseg,
                                              uint32_t nu, const BlockDesc &d, uint32_t bid, AT *y,
-                                             uint32_t nrows, double *partial, uint32_t *cnt)
+                                             uint32_t nrows, double *partial)
 {
     const int lane = lane_id();
     if (d.k != 0) {
-        reduce_long<AT>(sv, nu, d, bid, y, nrows, partial, cnt);
+        reduce_long<AT>(sv, nu, d, bid, y, nrows, partial);
         return;
     }
     uint32_t nseg = 0;
