@@ -209,6 +209,84 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
     reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
 }
 
+// AXT COM (tile height 1, exact CSR order, int_axt_com_h1,
+// intSpmv_main.c:1517-1566).  Slot s of the matrix is CSR entry s; sec[s] is a
+// head word (row << log | o) at the first slot of a row and at every slot
+// with s % bs == 0, and the segment it heads is slots s..s+o.  A head is
+// recognised by sec[s] != 0 or s % bs == 0: the second test also catches
+// row 0 with o == 0, whose word is 0 and which the reference kernel skips.
+// Each head lane labels its segment's slots with the row in LDS; each row is
+// then one lane's fused multiply-add chain over its slots in CSR order from
+// +0.0 -- the same bits as cf_CSR.
+constexpr uint32_t COM_UMAX = 256;
+
+template <typename VT, typename XT, typename AT>
+__global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
+    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ ax,
+    const uint32_t *__restrict__ sec, AT *__restrict__ y, uint32_t nrows, double *partial,
+    uint32_t bs, uint32_t lg)
+{
+    __shared__ AT s_v[WAVES][COM_UMAX];
+    __shared__ AT s_x[WAVES][COM_UMAX];
+    __shared__ uint32_t s_key[WAVES][COM_UMAX];
+    __shared__ uint16_t s_seg[WAVES][COM_UMAX + 1];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    if (bid >= nblocks) return;
+    const BlockDesc d = blocks[bid];
+    const uint32_t nu = d.uend - d.ubeg;
+    const uint32_t C = 2 * (lane & 3);
+    const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
+    AT acc = 0; // long rows: lane-strided fma partials
+    for (uint32_t Tb = T0; Tb < T1; Tb += 16) {
+        const uint32_t Tr = Tb + lane / 4, u0 = 8 * Tr + C;
+        const uint32_t T = Tr < T1 ? Tr : T1 - 1;
+        LoadT<VT> v0, v1;
+        LoadT<XT> x0, x1;
+        Pair<VT>::ld(ax + vidx<false>(T, 1, 0, C), v0, v1);
+        Pair<XT>::ld((const XT *)ax + xidx<false>(T, 1, 0, C), x0, x1);
+        const uint2 hw = *reinterpret_cast<const uint2 *>(sec + 8 * (size_t)T + C);
+        const bool ok0 = Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
+        const bool ok1 = Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
+        if (d.k != 0) {
+            if (ok0) acc = fma_acc<AT>((AT)v0, (AT)x0, acc);
+            if (ok1) acc = fma_acc<AT>((AT)v1, (AT)x1, acc);
+            continue;
+        }
+        if (ok0) {
+            s_v[w][u0 - d.ubeg] = (AT)v0;
+            s_x[w][u0 - d.ubeg] = (AT)x0;
+        }
+        if (ok1) {
+            s_v[w][u0 + 1 - d.ubeg] = (AT)v1;
+            s_x[w][u0 + 1 - d.ubeg] = (AT)x1;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t u = u0 + h, word = h ? hw.y : hw.x;
+            if ((h ? ok1 : ok0) && (word != 0 || u % bs == 0)) {
+                const uint32_t r = word >> lg, o = word & (bs - 1);
+                const uint32_t e = (u + o < d.uend ? u + o : d.uend - 1) - d.ubeg;
+                for (uint32_t i = u - d.ubeg; i <= e; i++) s_key[w][i] = r;
+            }
+        }
+    }
+    if (d.k != 0) {
+        finish_long<AT>(wave_sum<AT>(acc), d, bid, y, nrows, partial);
+        return;
+    }
+    wave_lds_sync();
+    const uint32_t nseg = find_segments(s_key[w], s_seg[w], nu);
+    for (uint32_t s = lane; s < nseg; s += WAVE) {
+        AT a = 0;
+        for (uint32_t i = s_seg[w][s], b = s_seg[w][s + 1]; i < b; i++)
+            a = fma_acc<AT>(s_v[w][i], s_x[w][i], a);
+        store_run<AT>(a, s, nseg, s_key[w], s_seg[w], d, y, nrows);
+    }
+    if (nseg == 0)
+        for (uint32_t z = d.rbeg + lane; z < d.rend && z < nrows; z += WAVE) y[z] = 0;
+}
+
 template <typename VT, typename XT, typename AT, bool PLANAR>
 static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, const uint32_t *sec,
                       void *y)
@@ -220,7 +298,14 @@ static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, con
     auto a = (const VT *)ax;
     auto x = (const XT *)xp;
     auto yy = (AT *)y;
-    if (th == 1) {
+    if (p->axt.mode == INTSPMV_COM) {
+        if constexpr (!PLANAR) {
+            axt_com_kernel<VT, XT, AT><<<grid, blk, 0, s>>>(p->d_blocks, nb, a, sec, yy, p->nrows,
+                                                           p->d_partial, p->axt.bs, p->axt.log);
+        } else {
+            return INTSPMV_ERR_UNSUPPORTED;
+        }
+    } else if (th == 1) {
         axt_h1_kernel<VT, XT, AT, PLANAR><<<grid, blk, 0, s>>>(
             p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial);
     } else {
@@ -255,7 +340,33 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
     if (!ctx || !desc || !plan || (desc->nrows && !con_host)) return INTSPMV_ERR_ARG;
     const intspmv_axt_desc &D = *desc;
     if (D.tileHW != 8 || D.tileH == 0) return INTSPMV_ERR_ARG;
-    if (D.mode != INTSPMV_NOC) return INTSPMV_ERR_UNSUPPORTED;
+    if (D.mode != INTSPMV_NOC && D.mode != INTSPMV_COM) return INTSPMV_ERR_ARG;
+    if (D.mode == INTSPMV_COM) {
+        // units are slots = CSR positions; con[r] = (p/8)*16 + p%8 with p the
+        // first CSR position of row r (getArraysLenAXT with positions = rl)
+        if (D.tileH != 1 || D.bs < 2 || D.bs > 512 || (D.bs & (D.bs - 1)) || (1u << D.log) != D.bs)
+            return INTSPMV_ERR_ARG;
+        if (D.colN != D.nnz || D.lenSEC != D.tileN * 8 || D.lenAX != D.tileN * 16 ||
+            D.nnz > D.lenSEC)
+            return INTSPMV_ERR_ARG;
+        uint32_t *uptr = (uint32_t *)malloc(((size_t)D.nrows + 1) * sizeof(uint32_t));
+        if (!uptr) return INTSPMV_ERR_NOMEM;
+        for (uint32_t r = 0; r < D.nrows; r++) {
+            uint32_t c = con_host[r];
+            uptr[r] = (c / 16) * 8 + c % 16;
+            if (c % 16 >= 8 || uptr[r] > D.nnz || (r && uptr[r] < uptr[r - 1])) {
+                free(uptr);
+                return INTSPMV_ERR_ARG;
+            }
+        }
+        uptr[D.nrows] = D.nnz;
+        int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, COM_UMAX, AXT_RMAX, 128, 1024,
+                                 plan);
+        free(uptr);
+        if (st) return st;
+        (*plan)->axt = D;
+        return INTSPMV_OK;
+    }
     const uint32_t th = D.tileH, ths = 8 * th;
     // unit start of each row from con (getArraysLenAXT, intSpmv_main.c:1130-1134):
     // con = T*2ths + R*16 + C; NOC rows start at brick-row 0 of a column.

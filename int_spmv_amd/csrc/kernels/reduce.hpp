@@ -66,6 +66,39 @@ __device__ __forceinline__ void reduce_long(const AT *sv, uint32_t nu, const Blo
     finish_long<AT>(wave_sum<AT>(acc), d, bid, y, nrows, partial);
 }
 
+// Finds the runs of equal keys (rows) of a normal block: sseg[0..nseg) are
+// the first units of the runs, sseg[nseg] = nu.  Returns nseg.
+__device__ __forceinline__ uint32_t find_segments(const uint32_t *sk, uint16_t *sseg, uint32_t nu)
+{
+    const int lane = lane_id();
+    uint32_t nseg = 0;
+    for (uint32_t b = 0; b < nu; b += WAVE) {
+        uint32_t p = b + lane;
+        bool head = p < nu && (p == 0 || sk[p] != sk[p - 1]);
+        uint64_t m = __ballot(head);
+        if (head) sseg[nseg + __popcll(m & lanemask_lt())] = (uint16_t)p;
+        nseg += __popcll(m);
+    }
+    if (lane == 0) sseg[nseg] = (uint16_t)nu;
+    wave_lds_sync();
+    return nseg;
+}
+
+// Writes the run result of row r and zeroes the empty rows of the block that
+// precede it (and, for the last run, follow it).
+template <typename AT>
+__device__ __forceinline__ void store_run(AT acc, uint32_t s, uint32_t nseg, const uint32_t *sk,
+                                          const uint16_t *sseg, const BlockDesc &d, AT *y,
+                                          uint32_t nrows)
+{
+    uint32_t r = sk[sseg[s]];
+    if (r < nrows) y[r] = acc;
+    uint32_t lo = (s == 0) ? d.rbeg : sk[sseg[s - 1]] + 1;
+    for (uint32_t z = lo; z < r && z < nrows; z++) y[z] = 0;
+    if (s == nseg - 1)
+        for (uint32_t z = r + 1; z < d.rend && z < nrows; z++) y[z] = 0;
+}
+
 // Keyed segmentation.  sseg needs room for nu + 1 entries.
 template <typename AT>
 __device__ __forceinline__ void reduce_keyed(const AT *sv, const uint32_t *sk, uint16_t *sseg,

@@ -190,3 +190,32 @@ def test_axt_repeat_is_deterministic(engine, mats):
         outs.append(_host(y, m.nrows))
     assert bitwise_equal(outs[0], outs[1]) and bitwise_equal(outs[1], outs[2])
     assert d.plan.info()["nsplit_rows"] > 0
+
+
+@pytest.mark.parametrize("bs", [2, 8, 128, 256, 512])
+def test_axt_com(engine, lap256, mats, tiny10, bs):
+    """COM is judged against CSR-naive (the reference's COM kernel drops row 0
+    segments whose head word is 0 and sums by prefix differences); rows that
+    fit one block are bit-identical to cf_CSR."""
+    fx, tm, tx = tiny10
+    cases = [("tiny10", tm, tx, O.cf_csr(tm.row, tm.col, tm.val, tx)),
+             ("lap256",) + lap256 + (O.cf_csr(lap256[0].row, lap256[0].col, lap256[0].val, lap256[1]),)]
+    cases += list(mats)
+    for name, m, x, yr in cases:
+        a = H.AXT(m, x, th=1, mode="COM", bs=bs)
+        d = D.DeviceAXT(engine, a)
+        y = _y(engine, m.nrows)
+        d.int_axt(y)
+        g = _host(y, m.nrows)
+        assert scaled(m, x, yr, g) <= TOL64, (name, bs)
+        short = np.diff(m.row.astype(np.int64)) <= 128
+        assert bitwise_equal(g[short], yr[short]), (name, bs)
+
+
+def test_axt_com_f32(engine, mats):
+    for name, m, x, yr in mats:
+        a = H.AXT(m, x, th=1, mode="COM", bs=128)
+        d = D.DeviceAXT(engine, a, val_type=D.F32, x_type=D.F32)
+        y = _y(engine, m.nrows, np.float32)
+        d.int_axt(y)
+        assert scaled(m, x, yr, _host(y, m.nrows)) <= TOL32, name
