@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -214,39 +215,41 @@ int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size
 }
 
 // Packs rows into blocks.  uptr[r]..uptr[r+1] are row r's units (monotone,
-// n+1 entries).  Rows with more than long_units units become long blocks,
-// split into ceil(len / long_piece) near-equal pieces when longer than
-// long_piece; other rows are packed greedily, at most umax units and rmax
-// rows per block.
+// n+1 entries).  Rows with at most long_units units are packed greedily into
+// normal blocks of at most umax units and rmax rows.  Longer rows become
+// long blocks of one wave each; rows longer than long_max are split into
+// ceil(len / long_max) near-equal pieces combined by the fix-up kernel.  The
+// long blocks come first, longest first, padded to whole XCD rounds of
+// workgroups; the normal blocks follow in row order.
 int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
-                    uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_piece,
+                    uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_max,
                     intspmv_plan **out)
 {
-    if (!out || !uptr || umax == 0 || rmax == 0 || long_units >= umax || long_piece == 0)
+    if (!out || !uptr || umax == 0 || rmax == 0 || long_units >= umax || long_max <= long_units)
         return INTSPMV_ERR_ARG;
     *out = nullptr;
     for (uint32_t r = 0; r < nrows; r++)
         if (uptr[r + 1] < uptr[r]) return INTSPMV_ERR_ARG;
-    std::vector<ispv::BlockDesc> blocks;
-    std::vector<ispv::SplitDesc> splits;
+    std::vector<ispv::BlockDesc> normal, lng;
+    std::vector<uint32_t> split_rows;
     uint32_t nlong = 0;
     uint32_t r = 0;
     while (r < nrows) {
         uint32_t len = uptr[r + 1] - uptr[r];
         if (len > long_units) {
-            uint32_t np = (len + long_piece - 1) / long_piece;
-            uint32_t b0 = (uint32_t)blocks.size();
+            uint32_t np = (len + long_max - 1) / long_max;
             if (np == 1) {
-                blocks.push_back({uptr[r], uptr[r + 1], r, r + 1, b0, 1, 0, 0});
+                lng.push_back({uptr[r], uptr[r + 1], r, r + 1, 0, 1, 0, 0});
                 nlong++;
             } else {
                 uint64_t per = (len + (uint64_t)np - 1) / np;
                 for (uint32_t q = 0; q < np; q++) {
                     uint64_t a = uptr[r] + q * per, b = a + per;
                     if (b > uptr[r + 1]) b = uptr[r + 1];
-                    blocks.push_back({(uint32_t)a, (uint32_t)b, r, r + 1, b0, np, 0, 0});
+                    // b0 is fixed up below once pieces have their final index
+                    lng.push_back({(uint32_t)a, (uint32_t)b, r, r + 1, q, np, 0, 0});
                 }
-                splits.push_back({r, b0, np, 0});
+                split_rows.push_back(r);
             }
             r++;
             continue;
@@ -258,8 +261,24 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
             units += l;
             r++;
         }
-        blocks.push_back({uptr[r0], uptr[r], r0, r, 0, 0, 0, 0});
+        normal.push_back({uptr[r0], uptr[r], r0, r, 0, 0, 0, 0});
     }
+    // longest first; pieces of one row stay together and in order
+    std::stable_sort(lng.begin(), lng.end(), [](const ispv::BlockDesc &a, const ispv::BlockDesc &b) {
+        uint32_t la = a.k > 1 ? 0xFFFFFFFFu - a.rbeg : a.uend - a.ubeg;
+        uint32_t lb = b.k > 1 ? 0xFFFFFFFFu - b.rbeg : b.uend - b.ubeg;
+        return la > lb;
+    });
+    std::vector<ispv::SplitDesc> splits;
+    for (size_t i = 0; i < lng.size(); i++)
+        if (lng[i].k > 1 && lng[i].b0 == 0) splits.push_back({lng[i].rbeg, (uint32_t)i, lng[i].k, 0});
+    for (auto &sp : splits)
+        for (uint32_t q = 0; q < sp.k; q++) lng[sp.b0 + q].b0 = sp.b0;
+    const uint32_t per_round = ispv::WAVES * ispv::NUM_XCD;
+    size_t nl = lng.empty() ? 0 : (lng.size() + per_round - 1) / per_round * per_round;
+    while (lng.size() < nl) lng.push_back({0, 0, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 1, 0, 0});
+    std::vector<ispv::BlockDesc> blocks(lng);
+    blocks.insert(blocks.end(), normal.begin(), normal.end());
     intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));
     if (!p) return INTSPMV_ERR_NOMEM;
     p->ctx = ctx;
@@ -267,6 +286,7 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
     p->nrows = nrows;
     p->nunits = nrows ? uptr[nrows] : 0;
     p->nblocks = (uint32_t)blocks.size();
+    p->nlong_wg = (uint32_t)(nl / ispv::WAVES);
     p->nsplit_rows = (uint32_t)splits.size();
     p->nlong_rows = nlong;
     p->umax = umax;

@@ -23,32 +23,53 @@ namespace ispv {
 constexpr uint32_t AXC_UMAX = 256;
 constexpr uint32_t AXC_RMAX = 256;
 constexpr uint32_t AXC_LONG = 64;
-constexpr uint32_t K1_LONG = 256; // steps per K1 work item
+constexpr uint32_t K1_LONG = 1024; // steps per K1 work item
 
 __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
-    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const double *__restrict__ ax,
-    const uint32_t *__restrict__ brp, double *__restrict__ y, uint32_t nrows, double *partial)
+    const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
+    const double *__restrict__ ax, const uint32_t *__restrict__ brp, double *__restrict__ y,
+    uint32_t nrows, double *partial)
 {
     __shared__ double s_val[WAVES][AXC_UMAX];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
     const uint32_t nu = d.uend - d.ubeg, q = lane & 3;
+    if (d.k != 0) {
+        // long row: per-lane product sums in registers, one wave tree
+        constexpr int UL = 8;
+        double acc = 0.0;
+        for (uint32_t base = 0; base < nu; base += 16 * UL) {
+            d2v v[UL], xx[UL];
+#pragma unroll
+            for (int j = 0; j < UL; j++) {
+                uint32_t t = base + j * 16 + lane / 4;
+                const double *b = ax + ((size_t)d.ubeg + (t < nu ? t : nu - 1)) * 16 + 2 * q;
+                v[j] = ntload(reinterpret_cast<const d2v *>(b));
+                xx[j] = ntload(reinterpret_cast<const d2v *>(b + 8));
+            }
+#pragma unroll
+            for (int j = 0; j < UL; j++)
+                if (base + j * 16 + lane / 4 < nu) acc = acc + v[j][0] * xx[j][0] + v[j][1] * xx[j][1];
+        }
+        finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial);
+        return;
+    }
     constexpr int U = 4;
     for (uint32_t base = 0; base < nu; base += 16 * U) {
-        double2 v[U], xx[U];
+        d2v v[U], xx[U];
 #pragma unroll
         for (int j = 0; j < U; j++) {
             uint32_t t = base + j * 16 + lane / 4;
             const double *b = ax + ((size_t)d.ubeg + (t < nu ? t : nu - 1)) * 16 + 2 * q;
-            v[j] = *reinterpret_cast<const double2 *>(b);
-            xx[j] = *reinterpret_cast<const double2 *>(b + 8);
+            v[j] = ntload(reinterpret_cast<const d2v *>(b));
+            xx[j] = ntload(reinterpret_cast<const d2v *>(b + 8));
         }
 #pragma unroll
         for (int j = 0; j < U; j++) {
             const bool ok = base + j * 16 + lane / 4 < nu;
-            double p0 = ok ? v[j].x * xx[j].x : 0.0, p1 = ok ? v[j].y * xx[j].y : 0.0;
+            double p0 = ok ? v[j][0] * xx[j][0] : 0.0, p1 = ok ? v[j][1] * xx[j][1] : 0.0;
             p0 = p0 + dpp_f64<DPP_QUAD_XOR2>(p0);
             p1 = p1 + dpp_f64<DPP_QUAD_XOR2>(p1);
             p0 = p0 + dpp_f64<DPP_QUAD_XOR1>(p0);
@@ -58,10 +79,6 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
         }
     }
     wave_lds_sync();
-    if (d.k != 0) {
-        reduce_long<double>(s_val[w], nu, d, bid, y, nrows, partial);
-        return;
-    }
     for (uint32_t r = d.rbeg + lane; r < d.rend; r += WAVE) {
         uint32_t a = brp[r] / 16 - d.ubeg, b = brp[r + 1] / 16 - d.ubeg;
         double sum = 0.0;
@@ -89,23 +106,24 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k1_kernel(
     if (have) d = items[it];
     double acc = 0.0;
     if (have) {
+        constexpr int U = 8;
         const uint32_t base = chp[d.c] + l;
-        uint32_t k = d.kbeg;
-        for (; k + 4 <= d.kend; k += 4) {
-            double v[4];
-            uint32_t cc[4];
+        for (uint32_t k = d.kbeg; k < d.kend; k += U) {
+            double v[U];
+            uint32_t cc[U];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                v[j] = val[base + 8 * (k + j)];
-                cc[j] = col[base + 8 * (k + j)];
+            for (int j = 0; j < U; j++) {
+                uint32_t kk = k + j < d.kend ? k + j : d.kend - 1;
+                v[j] = ntload(val + base + 8 * kk);
+                cc[j] = ntload(col + base + 8 * kk);
             }
-            double xv[4];
+            double xv[U];
 #pragma unroll
-            for (int j = 0; j < 4; j++) xv[j] = x[cc[j]];
+            for (int j = 0; j < U; j++) xv[j] = x[cc[j]];
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc = fma(v[j], xv[j], acc);
+            for (int j = 0; j < U; j++)
+                if (k + j < d.kend) acc = fma(v[j], xv[j], acc);
         }
-        for (; k < d.kend; k++) acc = fma(val[base + 8 * k], x[col[base + 8 * k]], acc);
     }
     const uint32_t s = d.c * 8 + l;
     if (!have) return;
@@ -158,7 +176,7 @@ extern "C" int intspmv_axc_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t
         }
         uptr[r] = brp_host[r] / 16;
     }
-    int st = ispv_plan_build(ctx, PLAN_AXC, nrows, uptr, AXC_UMAX, AXC_RMAX, AXC_LONG, AXC_UMAX,
+    int st = ispv_plan_build(ctx, PLAN_AXC, nrows, uptr, AXC_UMAX, AXC_RMAX, AXC_LONG, 1024,
                              plan);
     free(uptr);
     return st;
@@ -172,9 +190,8 @@ extern "C" int spmv_axc(intspmv_ctx *ctx, const intspmv_plan *plan, const double
     if (st) return st;
     if (plan->nblocks == 0) return INTSPMV_OK;
     if (!brp || !y || (plan->nunits && !ax)) return INTSPMV_ERR_ARG;
-    dim3 grid((plan->nblocks + WAVES - 1) / WAVES);
-    axc_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(plan->d_blocks, plan->nblocks, ax, brp,
-                                                        y, plan->nrows, plan->d_partial);
+    axc_kernel<<<ispv_grid(plan), BLOCK_THREADS, 0, ctx->stream>>>(
+        plan->d_blocks, plan->nlong_wg, plan->nblocks, ax, brp, y, plan->nrows, plan->d_partial);
     st = ispv_launch_check(ctx);
     if (st) return st;
     return ispv_split_fixup(plan, y, 0);

@@ -24,6 +24,9 @@ namespace ispv {
 
 constexpr uint32_t AXT_UMAX = 256;  // units (tiles or columns) per block
 constexpr uint32_t AXT_RMAX = 256;  // rows per block
+// A long row is streamed by one wave up to this many slots (128 KB of FP64
+// interleaved tiles); longer rows are split across waves.
+constexpr uint32_t LONG_SLOTS = 8192;
 
 template <typename T>
 struct Pair;
@@ -31,25 +34,25 @@ template <>
 struct Pair<double> {
     __device__ __forceinline__ static void ld(const double *p, double &a, double &b)
     {
-        double2 v = *reinterpret_cast<const double2 *>(p);
-        a = v.x;
-        b = v.y;
+        d2v v = ntload(reinterpret_cast<const d2v *>(p));
+        a = v[0];
+        b = v[1];
     }
 };
 template <>
 struct Pair<float> {
     __device__ __forceinline__ static void ld(const float *p, float &a, float &b)
     {
-        float2 v = *reinterpret_cast<const float2 *>(p);
-        a = v.x;
-        b = v.y;
+        f2v v = ntload(reinterpret_cast<const f2v *>(p));
+        a = v[0];
+        b = v[1];
     }
 };
 template <>
 struct Pair<uint16_t> { // bf16 storage
     __device__ __forceinline__ static void ld(const uint16_t *p, float &a, float &b)
     {
-        uint32_t v = *reinterpret_cast<const uint32_t *>(p);
+        uint32_t v = ntload(reinterpret_cast<const uint32_t *>(p));
         a = __uint_as_float(v << 16);
         b = __uint_as_float(v & 0xFFFF0000u);
     }
@@ -91,19 +94,53 @@ __device__ __forceinline__ size_t xidx(size_t T, uint32_t th, uint32_t R, uint32
     return PLANAR ? (T * th + R) * 8 + C : (T * th + R) * 16 + 8 + C;
 }
 
+// Long row of th==1 tiles [ubeg, uend): every lane keeps its own products'
+// running sum; one wave tree at the end.  UL steps of 16 tiles in flight.
+template <typename VT, typename XT, typename AT, bool PLANAR>
+__device__ __forceinline__ AT h1_long(const VT *av, const XT *xb, const BlockDesc &d)
+{
+    constexpr int UL = 8;
+    const int lane = lane_id();
+    const uint32_t nu = d.uend - d.ubeg, C = 2 * (lane & 3);
+    LaneSum<AT> acc;
+    for (uint32_t base = 0; base < nu; base += 16 * UL) {
+        LoadT<VT> v0[UL], v1[UL];
+        LoadT<XT> x0[UL], x1[UL];
+#pragma unroll
+        for (int j = 0; j < UL; j++) {
+            uint32_t t = base + j * 16 + lane / 4;
+            size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+            Pair<VT>::ld(av + vidx<PLANAR>(T, 1, 0, C), v0[j], v1[j]);
+            Pair<XT>::ld(xb + xidx<PLANAR>(T, 1, 0, C), x0[j], x1[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < UL; j++)
+            if (base + j * 16 + lane / 4 < nu) {
+                acc.add((AT)v0[j] * (AT)x0[j]);
+                acc.add((AT)v1[j] * (AT)x1[j]);
+            }
+    }
+    return wave_sum<AT>(acc.value());
+}
+
 template <typename VT, typename XT, typename AT, bool PLANAR>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_h1_kernel(
-    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ av,
-    const XT *__restrict__ axv, const uint32_t *__restrict__ sec, AT *__restrict__ y,
-    uint32_t nrows, double *partial)
+    const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
+    const VT *__restrict__ av, const XT *__restrict__ axv, const uint32_t *__restrict__ sec,
+    AT *__restrict__ y, uint32_t nrows, double *partial)
 {
     __shared__ AT s_val[WAVES][AXT_UMAX];
     __shared__ uint32_t s_key[WAVES][AXT_UMAX];
     __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
+    const XT *xb = PLANAR ? axv : (const XT *)av;
+    if (d.k != 0) {
+        finish_long<AT>(h1_long<VT, XT, AT, PLANAR>(av, xb, d), d, bid, y, nrows, partial);
+        return;
+    }
     const uint32_t nu = d.uend - d.ubeg;
     const uint32_t q = lane & 3, C = 2 * q;
     constexpr int U = 4; // steps of 16 tiles in flight per lane
@@ -118,8 +155,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_h1_kernel(
             uint32_t t = base + j * 16 + lane / 4;
             size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
             Pair<VT>::ld(av + vidx<PLANAR>(T, 1, 0, C), v0[j], v1[j]);
-            Pair<XT>::ld((PLANAR ? axv : (const XT *)av) + xidx<PLANAR>(T, 1, 0, C), x0[j], x1[j]);
-            key[j] = sec[T];
+            Pair<XT>::ld(xb + xidx<PLANAR>(T, 1, 0, C), x0[j], x1[j]);
+            key[j] = ntload(sec + T);
         }
 #pragma unroll
         for (int j = 0; j < U; j++) {
@@ -141,23 +178,58 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_h1_kernel(
     reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
 }
 
+// Column sums of the tile group starting at tile Tb for this lane's two
+// columns (fma over the th brick-rows in order).
+template <typename VT, typename XT, typename AT, bool PLANAR, int TH>
+__device__ __forceinline__ void noc_columns(const VT *av, const XT *xb, uint32_t T, uint32_t th,
+                                            uint32_t C, AT &a0, AT &a1)
+{
+    a0 = 0;
+    a1 = 0;
+    if constexpr (TH > 0) {
+        LoadT<VT> v0[TH], v1[TH];
+        LoadT<XT> x0[TH], x1[TH];
+#pragma unroll
+        for (int R = 0; R < TH; R++) {
+            Pair<VT>::ld(av + vidx<PLANAR>(T, TH, R, C), v0[R], v1[R]);
+            Pair<XT>::ld(xb + xidx<PLANAR>(T, TH, R, C), x0[R], x1[R]);
+        }
+#pragma unroll
+        for (int R = 0; R < TH; R++) {
+            a0 = fma_acc<AT>((AT)v0[R], (AT)x0[R], a0);
+            a1 = fma_acc<AT>((AT)v1[R], (AT)x1[R], a1);
+        }
+    } else {
+        for (uint32_t R = 0; R < th; R++) {
+            LoadT<VT> v0, v1;
+            LoadT<XT> x0, x1;
+            Pair<VT>::ld(av + vidx<PLANAR>(T, th, R, C), v0, v1);
+            Pair<XT>::ld(xb + xidx<PLANAR>(T, th, R, C), x0, x1);
+            a0 = fma_acc<AT>((AT)v0, (AT)x0, a0);
+            a1 = fma_acc<AT>((AT)v1, (AT)x1, a1);
+        }
+    }
+}
+
 template <typename VT, typename XT, typename AT, bool PLANAR, int TH>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
-    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ av,
-    const XT *__restrict__ axv, const uint32_t *__restrict__ sec, AT *__restrict__ y,
-    uint32_t nrows, double *partial, uint32_t th_rt)
+    const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
+    const VT *__restrict__ av, const XT *__restrict__ axv, const uint32_t *__restrict__ sec,
+    AT *__restrict__ y, uint32_t nrows, double *partial, uint32_t th_rt)
 {
     __shared__ AT s_val[WAVES][AXT_UMAX];
     __shared__ uint32_t s_key[WAVES][AXT_UMAX];
     __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
     const uint32_t th = TH ? TH : th_rt;
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
     const uint32_t nu = d.uend - d.ubeg;
     const uint32_t C = 2 * (lane & 3);
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
+    const XT *xb = PLANAR ? axv : (const XT *)av;
+    LaneSum<AT> lacc; // long rows
     for (uint32_t Tb = T0; Tb < T1; Tb += 16) {
         const uint32_t Tr = Tb + lane / 4, u0 = 8 * Tr + C;
         const bool ok0 = u0 >= d.ubeg && u0 < d.uend;
@@ -165,45 +237,26 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
         // lanes past the block re-read its last tile (results discarded) so
         // that the loads stay branch-free
         const uint32_t T = Tr < T1 ? Tr : T1 - 1;
-        AT a0 = 0, a1 = 0;
-        uint32_t k0 = 0, k1 = 0;
-        {
-            const XT *xb = PLANAR ? axv : (const XT *)av;
-            if constexpr (TH > 0) {
-                LoadT<VT> v0[TH], v1[TH];
-                LoadT<XT> x0[TH], x1[TH];
-#pragma unroll
-                for (int R = 0; R < TH; R++) {
-                    Pair<VT>::ld(av + vidx<PLANAR>(T, TH, R, C), v0[R], v1[R]);
-                    Pair<XT>::ld(xb + xidx<PLANAR>(T, TH, R, C), x0[R], x1[R]);
-                }
-#pragma unroll
-                for (int R = 0; R < TH; R++) {
-                    a0 = fma_acc<AT>((AT)v0[R], (AT)x0[R], a0);
-                    a1 = fma_acc<AT>((AT)v1[R], (AT)x1[R], a1);
-                }
-            } else {
-                for (uint32_t R = 0; R < th; R++) {
-                    LoadT<VT> v0, v1;
-                    LoadT<XT> x0, x1;
-                    Pair<VT>::ld(av + vidx<PLANAR>(T, th, R, C), v0, v1);
-                    Pair<XT>::ld(xb + xidx<PLANAR>(T, th, R, C), x0, x1);
-                    a0 = fma_acc<AT>((AT)v0, (AT)x0, a0);
-                    a1 = fma_acc<AT>((AT)v1, (AT)x1, a1);
-                }
-            }
-            uint2 kk = *reinterpret_cast<const uint2 *>(sec + 8 * (size_t)T + C);
-            k0 = kk.x;
-            k1 = kk.y;
+        AT a0, a1;
+        noc_columns<VT, XT, AT, PLANAR, TH>(av, xb, T, th, C, a0, a1);
+        if (d.k != 0) {
+            if (ok0) lacc.add(a0);
+            if (ok1) lacc.add(a1);
+            continue;
         }
+        const u2v kk = ntload(reinterpret_cast<const u2v *>(sec + 8 * (size_t)T + C));
         if (ok0) {
             s_val[w][u0 - d.ubeg] = a0;
-            s_key[w][u0 - d.ubeg] = k0;
+            s_key[w][u0 - d.ubeg] = kk[0];
         }
         if (ok1) {
             s_val[w][u0 + 1 - d.ubeg] = a1;
-            s_key[w][u0 + 1 - d.ubeg] = k1;
+            s_key[w][u0 + 1 - d.ubeg] = kk[1];
         }
+    }
+    if (d.k != 0) {
+        finish_long<AT>(wave_sum<AT>(lacc.value()), d, bid, y, nrows, partial);
+        return;
     }
     wave_lds_sync();
     reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
@@ -222,7 +275,7 @@ constexpr uint32_t COM_UMAX = 256;
 
 template <typename VT, typename XT, typename AT>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
-    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ ax,
+    const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks, const VT *__restrict__ ax,
     const uint32_t *__restrict__ sec, AT *__restrict__ y, uint32_t nrows, double *partial,
     uint32_t bs, uint32_t lg)
 {
@@ -231,7 +284,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
     __shared__ uint32_t s_key[WAVES][COM_UMAX];
     __shared__ uint16_t s_seg[WAVES][COM_UMAX + 1];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
     const uint32_t nu = d.uend - d.ubeg;
@@ -245,7 +298,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
         LoadT<XT> x0, x1;
         Pair<VT>::ld(ax + vidx<false>(T, 1, 0, C), v0, v1);
         Pair<XT>::ld((const XT *)ax + xidx<false>(T, 1, 0, C), x0, x1);
-        const uint2 hw = *reinterpret_cast<const uint2 *>(sec + 8 * (size_t)T + C);
+        const u2v hw = ntload(reinterpret_cast<const u2v *>(sec + 8 * (size_t)T + C));
         const bool ok0 = Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
         const bool ok1 = Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
         if (d.k != 0) {
@@ -263,7 +316,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
         }
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const uint32_t u = u0 + h, word = h ? hw.y : hw.x;
+            const uint32_t u = u0 + h, word = h ? hw[1] : hw[0];
             if ((h ? ok1 : ok0) && (word != 0 || u % bs == 0)) {
                 const uint32_t r = word >> lg, o = word & (bs - 1);
                 const uint32_t e = (u + o < d.uend ? u + o : d.uend - 1) - d.ubeg;
@@ -293,26 +346,27 @@ static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, con
 {
     const uint32_t nb = p->nblocks, th = p->axt.tileH;
     if (nb == 0) return INTSPMV_OK;
-    dim3 grid((nb + WAVES - 1) / WAVES), blk(BLOCK_THREADS);
+    dim3 grid(ispv_grid(p)), blk(BLOCK_THREADS);
+    const uint32_t nl = p->nlong_wg;
     hipStream_t s = p->ctx->stream;
     auto a = (const VT *)ax;
     auto x = (const XT *)xp;
     auto yy = (AT *)y;
     if (p->axt.mode == INTSPMV_COM) {
         if constexpr (!PLANAR) {
-            axt_com_kernel<VT, XT, AT><<<grid, blk, 0, s>>>(p->d_blocks, nb, a, sec, yy, p->nrows,
+            axt_com_kernel<VT, XT, AT><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows,
                                                            p->d_partial, p->axt.bs, p->axt.log);
         } else {
             return INTSPMV_ERR_UNSUPPORTED;
         }
     } else if (th == 1) {
         axt_h1_kernel<VT, XT, AT, PLANAR><<<grid, blk, 0, s>>>(
-            p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial);
+            p->d_blocks, nl, nb, a, x, sec, yy, p->nrows, p->d_partial);
     } else {
 #define ISPV_TH(N)                                                                            \
     case N:                                                                                   \
         axt_noc_kernel<VT, XT, AT, PLANAR, N><<<grid, blk, 0, s>>>(                           \
-            p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, th);            \
+            p->d_blocks, nl, nb, a, x, sec, yy, p->nrows, p->d_partial, th);        \
         break;
         switch (th) {
             ISPV_TH(2)
@@ -321,7 +375,7 @@ static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, con
             ISPV_TH(16)
         default:
             axt_noc_kernel<VT, XT, AT, PLANAR, 0><<<grid, blk, 0, s>>>(
-                p->d_blocks, nb, a, x, sec, yy, p->nrows, p->d_partial, th);
+                p->d_blocks, nl, nb, a, x, sec, yy, p->nrows, p->d_partial, th);
         }
 #undef ISPV_TH
     }
@@ -360,8 +414,8 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
             }
         }
         uptr[D.nrows] = D.nnz;
-        int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, COM_UMAX, AXT_RMAX, 128, 1024,
-                                 plan);
+        int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, COM_UMAX, AXT_RMAX, 128,
+                                 LONG_SLOTS, plan);
         free(uptr);
         if (st) return st;
         (*plan)->axt = D;
@@ -392,7 +446,7 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
     }
     uint32_t long_units = th == 1 ? 64 : (512 / th < 4 ? 4 : (512 / th > 64 ? 64 : 512 / th));
     int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, AXT_UMAX, AXT_RMAX, long_units,
-                             AXT_UMAX, plan);
+                             LONG_SLOTS / ths * (th == 1 ? 1 : 8), plan);
     free(uptr);
     if (st) return st;
     (*plan)->axt = D;

@@ -23,7 +23,8 @@ constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, workgroups dispatched rou
 // (CSR entries, AXC bricks, AXT tiles for th==1, AXT tile columns for th>1,
 // AXT COM slots).  Rows [rbeg, rend) own units [ubeg, uend).
 //   k == 0 : normal block, whole rows, each summed by one lane in unit order;
-//   k == 1 : one long row, reduced by a wave tree, written directly;
+//   k == 1 : one long row, streamed by one wave into per-lane registers and
+//            reduced by a wave tree, written directly;
 //   k >= 2 : piece of a long row split over k consecutive blocks b0..b0+k-1;
 //            each piece leaves its partial sum in the plan's scratch and the
 //            split fix-up kernel adds them in piece order.
@@ -40,7 +41,27 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t g, uint32_t n)
     return x * q + (x < r ? x : r) + l;
 }
 
+// Block index of this wave.  The first nlong_wg workgroups hold the long-row
+// blocks (longest first, dispatched round-robin over the XCDs so they start
+// early everywhere); the rest are the normal blocks, XCD-remapped.  nlong_wg
+// is a multiple of NUM_XCD so both sections keep hardware id % 8 == XCD.
+__device__ __forceinline__ uint32_t block_index(uint32_t nlong_wg)
+{
+    uint32_t g = blockIdx.x;
+    uint32_t lg = g < nlong_wg ? g : nlong_wg + xcd_remap(g - nlong_wg, gridDim.x - nlong_wg);
+    return lg * WAVES + threadIdx.x / WAVE;
+}
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// ---- streaming loads -------------------------------------------------------
+// Matrix arrays are read exactly once per product: nontemporal loads keep
+// them from displacing the gathered / reused data in L2 and the MALL.
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ T ntload(const T *p) { return __builtin_nontemporal_load(p); }
 
 // Orders this wave's LDS accesses before/after the call (no cross-wave sync).
 __device__ __forceinline__ void wave_lds_sync()
@@ -116,6 +137,7 @@ struct intspmv_plan {
     intspmv_ctx *ctx;
     int kind;
     uint32_t nrows, nunits, nblocks, nsplit_rows, nlong_rows;
+    uint32_t nlong_wg;  // workgroups of the long-row section (multiple of 8)
     uint32_t umax, rmax, long_units;
     ispv::BlockDesc *d_blocks;
     double *d_partial;          // [nblocks] (K1: [nitems * 8])
@@ -132,5 +154,10 @@ int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size
                      size_t nsplit);
 int ispv_split_fixup(const intspmv_plan *p, void *y, int y_is_f32);
 int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
-                    uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_piece,
+                    uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_max,
                     intspmv_plan **out);
+// grid size of a block plan
+static inline unsigned ispv_grid(const intspmv_plan *p)
+{
+    return p->nlong_wg + (p->nblocks - p->nlong_wg * ispv::WAVES + ispv::WAVES - 1) / ispv::WAVES;
+}

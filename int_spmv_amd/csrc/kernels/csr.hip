@@ -7,8 +7,9 @@
 //   values and columns with coalesced loads, gathers x[col] and stages both in
 //   LDS; phase 2 gives each row to one lane, which runs the same fma chain
 //   from LDS -- so every row of a normal block has exactly the ordered bits.
-//   Rows longer than 256 entries are reduced by a wave tree (and split over
-//   several waves beyond 512), within the FP64 tolerance.
+//   Rows longer than 256 entries are streamed by one wave each (split over
+//   several waves beyond 8192) and reduced by a wave tree, within the FP64
+//   tolerance.
 #include "common.hpp"
 #include "reduce.hpp"
 
@@ -17,6 +18,7 @@ namespace ispv {
 constexpr uint32_t CSR_UMAX = 512;
 constexpr uint32_t CSR_RMAX = 256;
 constexpr uint32_t CSR_LONG = 256;
+constexpr uint32_t CSR_LONG_MAX = 8192; // longest row one wave streams alone
 
 __global__ __launch_bounds__(BLOCK_THREADS) void csr_ordered_kernel(
     uint32_t nrows, const double *__restrict__ val, const uint32_t *__restrict__ col,
@@ -30,18 +32,39 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_ordered_kernel(
 }
 
 __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
-    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const double *__restrict__ val,
-    const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
-    const double *__restrict__ x, double *__restrict__ y, uint32_t nrows, double *partial)
+    const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
+    const double *__restrict__ val, const uint32_t *__restrict__ col,
+    const uint32_t *__restrict__ row, const double *__restrict__ x, double *__restrict__ y,
+    uint32_t nrows, double *partial)
 {
     __shared__ double s_v[WAVES][CSR_UMAX];
     __shared__ double s_x[WAVES][CSR_UMAX];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
+    const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
     const uint32_t nu = d.uend - d.ubeg;
     constexpr int U = 8;
+    if (d.k != 0) {
+        // long row: lane-strided fma chains in registers, one wave tree
+        double acc = 0.0;
+        for (uint32_t base = 0; base < nu; base += WAVE * U) {
+            double v[U];
+            uint32_t c[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                uint32_t i = base + j * WAVE + lane;
+                i = i < nu ? i : nu - 1;
+                v[j] = ntload(val + d.ubeg + i);
+                c[j] = ntload(col + d.ubeg + i);
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++)
+                if (base + j * WAVE + lane < nu) acc = fma(v[j], x[c[j]], acc);
+        }
+        finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial);
+        return;
+    }
     for (uint32_t base = 0; base < nu; base += WAVE * U) {
         double v[U];
         uint32_t c[U];
@@ -49,25 +72,22 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
         for (int j = 0; j < U; j++) {
             uint32_t i = base + j * WAVE + lane;
             i = i < nu ? i : nu - 1; // branch-free: re-read the last entry
-            v[j] = val[d.ubeg + i];
-            c[j] = col[d.ubeg + i];
+            v[j] = ntload(val + d.ubeg + i);
+            c[j] = ntload(col + d.ubeg + i);
         }
+        double xv[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) xv[j] = x[c[j]];
 #pragma unroll
         for (int j = 0; j < U; j++) {
             uint32_t i = base + j * WAVE + lane;
             if (i < nu) {
                 s_v[w][i] = v[j];
-                s_x[w][i] = x[c[j]];
+                s_x[w][i] = xv[j];
             }
         }
     }
     wave_lds_sync();
-    if (d.k != 0) {
-        double acc = 0.0;
-        for (uint32_t i = lane; i < nu; i += WAVE) acc = fma(s_v[w][i], s_x[w][i], acc);
-        finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial);
-        return;
-    }
     for (uint32_t r = d.rbeg + lane; r < d.rend; r += WAVE) {
         uint32_t a = row[r] - d.ubeg, b = row[r + 1] - d.ubeg;
         double acc = 0.0;
@@ -85,7 +105,7 @@ extern "C" int intspmv_csr_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t
 {
     if (!ctx || !plan || !row_host || row_host[0] != 0) return INTSPMV_ERR_ARG;
     return ispv_plan_build(ctx, PLAN_CSR, nrows, row_host, CSR_UMAX, CSR_RMAX, CSR_LONG,
-                           CSR_UMAX, plan);
+                           CSR_LONG_MAX, plan);
 }
 
 extern "C" int spmv_csr_ordered(intspmv_ctx *ctx, uint32_t nrows, const double *val,
@@ -109,9 +129,9 @@ extern "C" int spmv_csr(intspmv_ctx *ctx, const intspmv_plan *plan, const double
     if (st) return st;
     if (plan->nblocks == 0) return INTSPMV_OK;
     if (!row || !x || !y || (plan->nunits && (!val || !col))) return INTSPMV_ERR_ARG;
-    dim3 grid((plan->nblocks + WAVES - 1) / WAVES);
-    csr_block_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(
-        plan->d_blocks, plan->nblocks, val, col, row, x, y, plan->nrows, plan->d_partial);
+    csr_block_kernel<<<ispv_grid(plan), BLOCK_THREADS, 0, ctx->stream>>>(
+        plan->d_blocks, plan->nlong_wg, plan->nblocks, val, col, row, x, y, plan->nrows,
+        plan->d_partial);
     st = ispv_launch_check(ctx);
     if (st) return st;
     return ispv_split_fixup(plan, y, 0);

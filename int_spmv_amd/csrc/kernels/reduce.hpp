@@ -8,8 +8,8 @@
 //           1469-1470).  Rows of the block with no unit are written as 0.
 //   rowptr: the format's row pointer gives each row's units (CSR row[],
 //           AXC brp[]); one lane per row, same left-to-right order.
-//   long:   the block is one row: lane-strided partial sums and a wave tree;
-//           a row split over several blocks meets in the plan's scratch.
+//   long:   the block is one row: per-lane register sums and a wave tree; a
+//           row split over several blocks meets in the plan's scratch.
 #pragma once
 #include "common.hpp"
 
@@ -48,23 +48,21 @@ __device__ __forceinline__ void finish_long(AT total, const BlockDesc &d, uint32
     }
 }
 
-// Long row: all nu units of LDS array sv belong to row d.rbeg.
+// Per-lane running sum of a long row.  FP64 adds plainly; FP32 keeps a
+// Kahan-Babuska-Neumaier compensation term, so rows of thousands of entries
+// keep FP32 accumulation accurate.
 template <typename AT>
-__device__ __forceinline__ void reduce_long(const AT *sv, uint32_t nu, const BlockDesc &d,
-                                            uint32_t bid, AT *y, uint32_t nrows,
-                                            double *partial)
-{
-    const int lane = lane_id();
-    AT acc = 0;
-    if constexpr (sizeof(AT) == 4) {
-        Neumaier k;
-        for (uint32_t i = lane; i < nu; i += WAVE) k.add(sv[i]);
-        acc = k.value();
-    } else {
-        for (uint32_t i = lane; i < nu; i += WAVE) acc = acc + sv[i];
-    }
-    finish_long<AT>(wave_sum<AT>(acc), d, bid, y, nrows, partial);
-}
+struct LaneSum {
+    AT s = 0;
+    __device__ __forceinline__ void add(AT v) { s = s + v; }
+    __device__ __forceinline__ AT value() const { return s; }
+};
+template <>
+struct LaneSum<float> {
+    Neumaier k;
+    __device__ __forceinline__ void add(float v) { k.add(v); }
+    __device__ __forceinline__ float value() const { return k.value(); }
+};
 
 // Finds the runs of equal keys (rows) of a normal block: sseg[0..nseg) are
 // the first units of the runs, sseg[nseg] = nu.  Returns nseg.
@@ -106,10 +104,6 @@ __device__ __forceinline__ void reduce_keyed(const AT *sv, const uint32_t *sk, u
                                              uint32_t nrows, double *partial)
 {
     const int lane = lane_id();
-    if (d.k != 0) {
-        reduce_long<AT>(sv, nu, d, bid, y, nrows, partial);
-        return;
-    }
     uint32_t nseg = 0;
     for (uint32_t b = 0; b < nu; b += WAVE) {
         uint32_t p = b + lane;

@@ -13,7 +13,7 @@ import oracle as O
 from int_spmv_amd import device as D
 from int_spmv_amd import host as H
 
-from .util import bitwise_equal, csr_with_empty_rows, scaled
+from .util import bitwise_equal, csr_long_rows, csr_with_empty_rows, scaled
 
 pytestmark = pytest.mark.gpu
 
@@ -38,6 +38,7 @@ def _matrices():
     yield "powerlaw16k", H.CSR.powerlaw(n=1 << 14, target_nnz=460_000, maxlen=8192, seed=1)
     yield "empty_rows", csr_with_empty_rows()
     yield "banded", H.CSR.banded(5000, 4, 64, seed=2)
+    yield "long_rows", csr_long_rows()
 
 
 @pytest.fixture(scope="module")
@@ -115,7 +116,7 @@ def test_k1(engine, lap256, mats, tiny10):
         d.int_k1p(xd, y2)
         ref_res = O.int_k1(k.chunkNum, k.val, k.col, k.chp, k.nmc, x)
         g_res = _host(res, k.chunkNum * 8)
-        split = np.repeat(k.nmc > 256, 8)
+        split = np.repeat(k.nmc > 1024, 8)
         assert bitwise_equal(g_res[~split], ref_res[~split]), name
         yr = O.cf_csr(m.row, m.col, m.val, x)
         assert scaled(m, x, yr, _host(y, m.nrows)) <= TOL64, name
@@ -180,7 +181,7 @@ def test_axt_precision(engine, mats, th, vt, xt, tol):
 
 
 def test_axt_repeat_is_deterministic(engine, mats):
-    name, m, x, yr = mats[0]
+    name, m, x, yr = mats[3]
     a = H.AXT(m, x, th=1)
     d = D.DeviceAXT(engine, a)
     outs = []

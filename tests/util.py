@@ -33,3 +33,16 @@ def bitwise_equal(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return a.shape == b.shape and bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+def csr_long_rows(n=30000, seed=5):
+    """Mostly short rows plus rows of 9,000-30,000 entries, longer than one
+    wave streams alone in every format, so split rows are exercised."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 12, size=n)
+    lens[10], lens[11], lens[5000], lens[n - 1] = 20000, 9000, 12000, 30000
+    row = np.zeros(n + 1, np.uint32)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.integers(0, n, size=l)) for l in lens]).astype(np.uint32)
+    val = rng.standard_normal(len(col))
+    return H.CSR.from_arrays(row, col, val)
