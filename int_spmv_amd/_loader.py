@@ -4,7 +4,9 @@ from __future__ import annotations
 import ctypes
 import os
 
-LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+# INTSPMV_LIB_DIR points at another build of the libraries (e.g. a tuning variant)
+LIBDIR = os.environ.get("INTSPMV_LIB_DIR") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "lib")
 
 
 class NativeLibraryMissing(RuntimeError):

@@ -51,12 +51,13 @@ def build_host() -> str:
     return out
 
 
-def build_device() -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
+def build_device(outdir: str = LIBDIR, defines: tuple = ()) -> str:
+    """defines: extra -D flags overriding kernel tunables (ISPV_* macros)."""
+    os.makedirs(outdir, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "kernels", "*.hip")))
-    out = os.path.join(LIBDIR, "libintspmv.so")
-    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{os.path.join(CSRC, 'kernels')}", *srcs,
-          "-o", out + ".tmp"])
+    out = os.path.join(outdir, "libintspmv.so")
+    _run([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], f"-I{INCLUDE}",
+          f"-I{os.path.join(CSRC, 'kernels')}", *srcs, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -20,7 +20,7 @@
 
 namespace ispv {
 
-constexpr uint32_t AXC_UMAX = 256;
+constexpr uint32_t AXC_UMAX = 128;
 constexpr uint32_t AXC_RMAX = 256;
 constexpr uint32_t AXC_LONG = 64;
 constexpr uint32_t K1_LONG = 1024; // steps per K1 work item
@@ -35,47 +35,49 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
     const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
-    const uint32_t nu = d.uend - d.ubeg, q = lane & 3;
+    // full-line mapping (as the AXT th==1 kernel): each load instruction
+    // covers 8 whole 128-byte bricks; part p = l%8 holds values 2p,2p+1 (p<4)
+    // or x (p>=4), and a DPP row shift pairs them
+    const uint32_t nu = d.uend - d.ubeg, p = lane & 7;
+    constexpr int U = 8;
     if (d.k != 0) {
-        // long row: per-lane product sums in registers, one wave tree
-        constexpr int UL = 8;
         double acc = 0.0;
-        for (uint32_t base = 0; base < nu; base += 16 * UL) {
-            d2v v[UL], xx[UL];
+        for (uint32_t base = 0; base < nu; base += 8 * 2 * U) {
+            d2v v[2 * U];
 #pragma unroll
-            for (int j = 0; j < UL; j++) {
-                uint32_t t = base + j * 16 + lane / 4;
-                const double *b = ax + ((size_t)d.ubeg + (t < nu ? t : nu - 1)) * 16 + 2 * q;
-                v[j] = ntload(reinterpret_cast<const d2v *>(b));
-                xx[j] = ntload(reinterpret_cast<const d2v *>(b + 8));
+            for (int j = 0; j < 2 * U; j++) {
+                uint32_t t = base + j * 8 + lane / 8;
+                size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+                v[j] = ntload(reinterpret_cast<const d2v *>(ax + T * 16 + 2 * p));
             }
 #pragma unroll
-            for (int j = 0; j < UL; j++)
-                if (base + j * 16 + lane / 4 < nu) acc = acc + v[j][0] * xx[j][0] + v[j][1] * xx[j][1];
+            for (int j = 0; j < 2 * U; j++) {
+                double x0 = dpp_f64<DPP_ROW_SHL4>(v[j][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[j][1]);
+                if (base + j * 8 + lane / 8 < nu && p < 4) acc = acc + v[j][0] * x0 + v[j][1] * x1;
+            }
         }
         finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial);
         return;
     }
-    constexpr int U = 4;
-    for (uint32_t base = 0; base < nu; base += 16 * U) {
-        d2v v[U], xx[U];
+    for (uint32_t base = 0; base < nu; base += 8 * U) {
+        d2v v[U];
 #pragma unroll
         for (int j = 0; j < U; j++) {
-            uint32_t t = base + j * 16 + lane / 4;
-            const double *b = ax + ((size_t)d.ubeg + (t < nu ? t : nu - 1)) * 16 + 2 * q;
-            v[j] = ntload(reinterpret_cast<const d2v *>(b));
-            xx[j] = ntload(reinterpret_cast<const d2v *>(b + 8));
+            uint32_t t = base + j * 8 + lane / 8;
+            size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+            v[j] = ntload(reinterpret_cast<const d2v *>(ax + T * 16 + 2 * p));
         }
 #pragma unroll
         for (int j = 0; j < U; j++) {
-            const bool ok = base + j * 16 + lane / 4 < nu;
-            double p0 = ok ? v[j][0] * xx[j][0] : 0.0, p1 = ok ? v[j][1] * xx[j][1] : 0.0;
+            const bool ok = base + j * 8 + lane / 8 < nu && p < 4;
+            double x0 = dpp_f64<DPP_ROW_SHL4>(v[j][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[j][1]);
+            double p0 = ok ? v[j][0] * x0 : 0.0, p1 = ok ? v[j][1] * x1 : 0.0;
             p0 = p0 + dpp_f64<DPP_QUAD_XOR2>(p0);
             p1 = p1 + dpp_f64<DPP_QUAD_XOR2>(p1);
             p0 = p0 + dpp_f64<DPP_QUAD_XOR1>(p0);
             p1 = p1 + dpp_f64<DPP_QUAD_XOR1>(p1);
-            uint32_t t = base + j * 16 + lane / 4;
-            if (t < nu && q == 0) s_val[w][t] = p0 + p1;
+            uint32_t t = base + j * 8 + lane / 8;
+            if (t < nu && p == 0) s_val[w][t] = p0 + p1;
         }
     }
     wave_lds_sync();

@@ -22,11 +22,23 @@
 
 namespace ispv {
 
-constexpr uint32_t AXT_UMAX = 256;  // units (tiles or columns) per block
+#ifndef ISPV_AXT_UMAX
+#define ISPV_AXT_UMAX 128
+#endif
+#ifndef ISPV_H1_STEPS
+#define ISPV_H1_STEPS 4
+#endif
+constexpr uint32_t AXT_UMAX = ISPV_AXT_UMAX; // units (tiles or columns) per block
 constexpr uint32_t AXT_RMAX = 256;  // rows per block
 // A long row is streamed by one wave up to this many slots (128 KB of FP64
 // interleaved tiles); longer rows are split across waves.
-constexpr uint32_t LONG_SLOTS = 8192;
+#ifndef ISPV_LONG_SLOTS
+#define ISPV_LONG_SLOTS 8192
+#endif
+#ifndef ISPV_H1_LONG
+#define ISPV_H1_LONG 64
+#endif
+constexpr uint32_t LONG_SLOTS = ISPV_LONG_SLOTS;
 
 template <typename T>
 struct Pair;
@@ -96,11 +108,34 @@ __device__ __forceinline__ size_t xidx(size_t T, uint32_t th, uint32_t R, uint32
 
 // Long row of th==1 tiles [ubeg, uend): every lane keeps its own products'
 // running sum; one wave tree at the end.  UL steps of 16 tiles in flight.
+#ifndef ISPV_H1_LONG_STEPS
+#define ISPV_H1_LONG_STEPS 8
+#endif
 template <typename VT, typename XT, typename AT, bool PLANAR>
 __device__ __forceinline__ AT h1_long(const VT *av, const XT *xb, const BlockDesc &d)
 {
-    constexpr int UL = 8;
+    constexpr int UL = ISPV_H1_LONG_STEPS;
     const int lane = lane_id();
+    if constexpr (!PLANAR && sizeof(VT) == 8) {
+        // full-line mapping as in the normal path: 8 tiles per load
+        const uint32_t nu = d.uend - d.ubeg, p = lane & 7;
+        double acc = 0.0;
+        for (uint32_t base = 0; base < nu; base += 8 * 2 * UL) {
+            d2v v[2 * UL];
+#pragma unroll
+            for (int j = 0; j < 2 * UL; j++) {
+                uint32_t t = base + j * 8 + lane / 8;
+                size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+                v[j] = ntload(reinterpret_cast<const d2v *>(av + T * 16 + 2 * p));
+            }
+#pragma unroll
+            for (int j = 0; j < 2 * UL; j++) {
+                double x0 = dpp_f64<DPP_ROW_SHL4>(v[j][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[j][1]);
+                if (base + j * 8 + lane / 8 < nu && p < 4) acc = acc + v[j][0] * x0 + v[j][1] * x1;
+            }
+        }
+        return (AT)wave_sum_f64(acc);
+    }
     const uint32_t nu = d.uend - d.ubeg, C = 2 * (lane & 3);
     LaneSum<AT> acc;
     for (uint32_t base = 0; base < nu; base += 16 * UL) {
@@ -123,8 +158,13 @@ __device__ __forceinline__ AT h1_long(const VT *av, const XT *xb, const BlockDes
     return wave_sum<AT>(acc.value());
 }
 
+#ifdef ISPV_WAVES_PER_EU
+#define ISPV_OCC __attribute__((amdgpu_waves_per_eu(ISPV_WAVES_PER_EU)))
+#else
+#define ISPV_OCC
+#endif
 template <typename VT, typename XT, typename AT, bool PLANAR>
-__global__ __launch_bounds__(BLOCK_THREADS) void axt_h1_kernel(
+__global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
     const VT *__restrict__ av, const XT *__restrict__ axv, const uint32_t *__restrict__ sec,
     AT *__restrict__ y, uint32_t nrows, double *partial)
@@ -142,8 +182,45 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_h1_kernel(
         return;
     }
     const uint32_t nu = d.uend - d.ubeg;
+    if constexpr (!PLANAR && sizeof(VT) == 8) {
+        // FP64 reference layout: each load instruction covers 8 whole 128-byte
+        // tiles (1 KB, full cache lines).  Lane part p = l%8 holds values
+        // 2p,2p+1 (p<4) or x 2(p-4),2(p-4)+1 (p>=4); one DPP row shift brings
+        // the x pair to the value lanes, whose quads then form the tree.
+        const uint32_t p = lane & 7;
+        constexpr int U = 2 * ISPV_H1_STEPS;
+        for (uint32_t base = 0; base < nu; base += 8 * U) {
+            d2v v[U];
+            uint32_t key[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                uint32_t t = base + j * 8 + lane / 8;
+                size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+                v[j] = ntload(reinterpret_cast<const d2v *>(av + T * 16 + 2 * p));
+                key[j] = ntload(sec + T);
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const bool ok = base + j * 8 + lane / 8 < nu && p < 4;
+                double x0 = dpp_f64<DPP_ROW_SHL4>(v[j][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[j][1]);
+                double p0 = ok ? v[j][0] * x0 : 0.0, p1 = ok ? v[j][1] * x1 : 0.0;
+                p0 = p0 + dpp_f64<DPP_QUAD_XOR2>(p0);
+                p1 = p1 + dpp_f64<DPP_QUAD_XOR2>(p1);
+                p0 = p0 + dpp_f64<DPP_QUAD_XOR1>(p0);
+                p1 = p1 + dpp_f64<DPP_QUAD_XOR1>(p1);
+                uint32_t t = base + j * 8 + lane / 8;
+                if (t < nu && p == 0) {
+                    s_val[w][t] = (AT)(p0 + p1);
+                    s_key[w][t] = key[j];
+                }
+            }
+        }
+        wave_lds_sync();
+        reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
+        return;
+    }
     const uint32_t q = lane & 3, C = 2 * q;
-    constexpr int U = 4; // steps of 16 tiles in flight per lane
+    constexpr int U = ISPV_H1_STEPS; // steps of 16 tiles in flight per lane
     for (uint32_t base = 0; base < nu; base += 16 * U) {
         LoadT<VT> v0[U], v1[U];
         LoadT<XT> x0[U], x1[U];
@@ -291,28 +368,31 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
     const uint32_t C = 2 * (lane & 3);
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
     AT acc = 0; // long rows: lane-strided fma partials
-    for (uint32_t Tb = T0; Tb < T1; Tb += 16) {
-        const uint32_t Tr = Tb + lane / 4, u0 = 8 * Tr + C;
+    // full-line mapping: 8 whole tiles per load, lane part p = l%8; values of
+    // slots 2p, 2p+1 (p < 4) meet their x through a DPP row shift
+    const uint32_t p = lane & 7, Cq = 2 * (p & 3);
+    (void)C;
+    for (uint32_t Tb = T0; Tb < T1; Tb += 8) {
+        const uint32_t Tr = Tb + lane / 8, u0 = 8 * Tr + Cq;
         const uint32_t T = Tr < T1 ? Tr : T1 - 1;
         LoadT<VT> v0, v1;
-        LoadT<XT> x0, x1;
-        Pair<VT>::ld(ax + vidx<false>(T, 1, 0, C), v0, v1);
-        Pair<XT>::ld((const XT *)ax + xidx<false>(T, 1, 0, C), x0, x1);
-        const u2v hw = ntload(reinterpret_cast<const u2v *>(sec + 8 * (size_t)T + C));
-        const bool ok0 = Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
-        const bool ok1 = Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
+        Pair<VT>::ld(ax + (size_t)T * 16 + 2 * p, v0, v1);
+        const AT x0 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v0), x1 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v1);
+        const bool ok0 = p < 4 && Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
+        const bool ok1 = p < 4 && Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
         if (d.k != 0) {
-            if (ok0) acc = fma_acc<AT>((AT)v0, (AT)x0, acc);
-            if (ok1) acc = fma_acc<AT>((AT)v1, (AT)x1, acc);
+            if (ok0) acc = fma_acc<AT>((AT)v0, x0, acc);
+            if (ok1) acc = fma_acc<AT>((AT)v1, x1, acc);
             continue;
         }
+        const u2v hw = ntload(reinterpret_cast<const u2v *>(sec + 8 * (size_t)T + Cq));
         if (ok0) {
             s_v[w][u0 - d.ubeg] = (AT)v0;
-            s_x[w][u0 - d.ubeg] = (AT)x0;
+            s_x[w][u0 - d.ubeg] = x0;
         }
         if (ok1) {
             s_v[w][u0 + 1 - d.ubeg] = (AT)v1;
-            s_x[w][u0 + 1 - d.ubeg] = (AT)x1;
+            s_x[w][u0 + 1 - d.ubeg] = x1;
         }
 #pragma unroll
         for (int h = 0; h < 2; h++) {
@@ -340,6 +420,75 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
         for (uint32_t z = d.rbeg + lane; z < d.rend && z < nrows; z += WAVE) y[z] = 0;
 }
 
+// th > 1, FP64 reference layout, full-line mapping: lane (tile Tb + l/8,
+// part p = l%8) reads 16 bytes of each brick-row of its tile, so every load
+// instruction covers 8 whole 128-byte brick-rows; a DPP row shift brings the
+// x pair to the value lanes (p < 4), which run the fma chains of columns
+// 2p, 2p+1 over the brick-rows in order.
+template <int TH>
+__global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
+    const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
+    const double *__restrict__ ax, const uint32_t *__restrict__ sec, double *__restrict__ y,
+    uint32_t nrows, double *partial, uint32_t th_rt)
+{
+    __shared__ double s_val[WAVES][AXT_UMAX];
+    __shared__ uint32_t s_key[WAVES][AXT_UMAX];
+    __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
+    const uint32_t th = TH ? TH : th_rt;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const uint32_t bid = block_index(nlong_wg);
+    if (bid >= nblocks) return;
+    const BlockDesc d = blocks[bid];
+    const uint32_t nu = d.uend - d.ubeg, p = lane & 7, C = 2 * (p & 3);
+    const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
+    double lacc = 0.0; // long rows
+    for (uint32_t Tb = T0; Tb < T1; Tb += 8) {
+        const uint32_t Tr = Tb + lane / 8, u0 = 8 * Tr + C;
+        const bool ok0 = p < 4 && Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
+        const bool ok1 = p < 4 && Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
+        const size_t T = Tr < T1 ? Tr : T1 - 1;
+        double a0 = 0.0, a1 = 0.0;
+        if constexpr (TH > 0) {
+            d2v v[TH];
+#pragma unroll
+            for (int R = 0; R < TH; R++) v[R] = ntload(reinterpret_cast<const d2v *>(ax + (T * TH + R) * 16 + 2 * p));
+#pragma unroll
+            for (int R = 0; R < TH; R++) {
+                double x0 = dpp_f64<DPP_ROW_SHL4>(v[R][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[R][1]);
+                a0 = fma(v[R][0], x0, a0);
+                a1 = fma(v[R][1], x1, a1);
+            }
+        } else {
+            for (uint32_t R = 0; R < th; R++) {
+                d2v v = ntload(reinterpret_cast<const d2v *>(ax + (T * th + R) * 16 + 2 * p));
+                double x0 = dpp_f64<DPP_ROW_SHL4>(v[0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[1]);
+                a0 = fma(v[0], x0, a0);
+                a1 = fma(v[1], x1, a1);
+            }
+        }
+        if (d.k != 0) {
+            if (ok0) lacc = lacc + a0;
+            if (ok1) lacc = lacc + a1;
+            continue;
+        }
+        const u2v kk = ntload(reinterpret_cast<const u2v *>(sec + 8 * T + C));
+        if (ok0) {
+            s_val[w][u0 - d.ubeg] = a0;
+            s_key[w][u0 - d.ubeg] = kk[0];
+        }
+        if (ok1) {
+            s_val[w][u0 + 1 - d.ubeg] = a1;
+            s_key[w][u0 + 1 - d.ubeg] = kk[1];
+        }
+    }
+    if (d.k != 0) {
+        finish_long<double>(wave_sum_f64(lacc), d, bid, y, nrows, partial);
+        return;
+    }
+    wave_lds_sync();
+    reduce_keyed<double>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
+}
+
 template <typename VT, typename XT, typename AT, bool PLANAR>
 static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, const uint32_t *sec,
                       void *y)
@@ -362,6 +511,14 @@ static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, con
     } else if (th == 1) {
         axt_h1_kernel<VT, XT, AT, PLANAR><<<grid, blk, 0, s>>>(
             p->d_blocks, nl, nb, a, x, sec, yy, p->nrows, p->d_partial);
+    } else if constexpr (!PLANAR && sizeof(VT) == 8) {
+        switch (th) {
+        case 2: axt_noc_fl_kernel<2><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;
+        case 4: axt_noc_fl_kernel<4><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;
+        case 8: axt_noc_fl_kernel<8><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;
+        case 16: axt_noc_fl_kernel<16><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;
+        default: axt_noc_fl_kernel<0><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th);
+        }
     } else {
 #define ISPV_TH(N)                                                                            \
     case N:                                                                                   \
@@ -444,7 +601,7 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
         free(uptr);
         return INTSPMV_ERR_ARG;
     }
-    uint32_t long_units = th == 1 ? 64 : (512 / th < 4 ? 4 : (512 / th > 64 ? 64 : 512 / th));
+    uint32_t long_units = th == 1 ? ISPV_H1_LONG : (512 / th < 4 ? 4 : (512 / th > 64 ? 64 : 512 / th));
     int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, AXT_UMAX, AXT_RMAX, long_units,
                              LONG_SLOTS / ths * (th == 1 ? 1 : 8), plan);
     free(uptr);

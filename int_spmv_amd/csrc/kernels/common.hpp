@@ -49,7 +49,9 @@ __device__ __forceinline__ uint32_t block_index(uint32_t nlong_wg)
 {
     uint32_t g = blockIdx.x;
     uint32_t lg = g < nlong_wg ? g : nlong_wg + xcd_remap(g - nlong_wg, gridDim.x - nlong_wg);
-    return lg * WAVES + threadIdx.x / WAVE;
+    // wave-uniform by construction; readfirstlane lets the block descriptor
+    // come in through scalar loads
+    return __builtin_amdgcn_readfirstlane(lg * WAVES + threadIdx.x / WAVE);
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -85,8 +87,16 @@ __device__ __forceinline__ float dpp_f32(float v)
 {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
 }
+template <typename T, int CTRL>
+__device__ __forceinline__ T dpp_any(T v)
+{
+    if constexpr (sizeof(T) == 8) return dpp_f64<CTRL>(v);
+    else return dpp_f32<CTRL>(v);
+}
 constexpr int DPP_QUAD_XOR1 = 0xB1; // quad_perm [1,0,3,2]
 constexpr int DPP_QUAD_XOR2 = 0x4E; // quad_perm [2,3,0,1]
+constexpr int DPP_ROW_SHL4 = 0x104; // lane i reads lane i+4 of its row of 16
+constexpr int DPP_ROW_SHL2 = 0x102; // lane i reads lane i+2 of its row of 16
 
 __device__ __forceinline__ double shfl_xor_f64(double v, int m) { return __shfl_xor(v, m, WAVE); }
 __device__ __forceinline__ float shfl_xor_f32(float v, int m) { return __shfl_xor(v, m, WAVE); }
