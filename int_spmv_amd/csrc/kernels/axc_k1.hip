@@ -8,8 +8,8 @@
 // K1 (int_k1 / int_k1p, intSpmv_main.c:943-1066): chunk c holds 8 rows of the
 //   length-sorted order, element k of lane l at chp[c] + 8k + l.  Eight lanes
 //   per chunk, one fma chain per lane in k order (the reference's lane-wise
-//   vtSum).  Chunks with more than K1_LONG steps are cut into pieces that
-//   meet in the plan scratch.  The result goes to res[8c + l] (int_k1) or is
+//   vtSum).  Chunks with more than K1_WIDE steps get a whole wave each (and
+//   are cut into K1_PIECE-step pieces that meet in the plan scratch).  The result goes to res[8c + l] (int_k1) or is
 //   scattered to y[permi[8c + l]] for the valid lanes (int_k1p).
 #include <stdlib.h>
 
@@ -23,7 +23,8 @@ namespace ispv {
 constexpr uint32_t AXC_UMAX = 128;
 constexpr uint32_t AXC_RMAX = 256;
 constexpr uint32_t AXC_LONG = 64;
-constexpr uint32_t K1_LONG = 1024; // steps per K1 work item
+constexpr uint32_t K1_WIDE = 32;   // chunks with more steps get a whole wave
+constexpr uint32_t K1_PIECE = 512; // steps per wide item (longer chunks split)
 
 __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
@@ -94,21 +95,53 @@ struct K1Item {
     uint32_t c, kbeg, kend, b0, k, pad0, pad1, pad2;
 };
 
+// Two sections of waves.  The first nwide waves take one item each with all
+// 64 lanes: lane (l%8 = row of the chunk, l/8 = step slice) runs the steps
+// k = kbeg + l/8 (mod 8), so every load instruction reads 64 consecutive
+// entries, and the eight slices are added by a tree at the end (long chunks,
+// where eight lanes alone would serialise thousands of dependent gathers).
+// The remaining waves take eight short items each, eight lanes per item, one
+// fma chain per row in step order -- the reference's lane-wise vtSum.
 __global__ __launch_bounds__(BLOCK_THREADS) void k1_kernel(
-    const K1Item *__restrict__ items, uint32_t nitems, const double *__restrict__ val,
-    const uint32_t *__restrict__ col, const uint32_t *__restrict__ chp,
-    const uint32_t *__restrict__ permi, const double *__restrict__ x, double *__restrict__ y,
-    uint32_t nrows, double *partial)
+    const K1Item *__restrict__ items, uint32_t nitems, uint32_t nwide,
+    const double *__restrict__ val, const uint32_t *__restrict__ col,
+    const uint32_t *__restrict__ chp, const uint32_t *__restrict__ permi,
+    const double *__restrict__ x, double *__restrict__ y, uint32_t nrows, double *partial)
 {
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
-    const uint32_t wid = xcd_remap(blockIdx.x, gridDim.x) * WAVES + w;
-    const uint32_t it = wid * 8 + lane / 8, l = lane % 8;
-    bool have = it < nitems;
-    K1Item d = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (have) d = items[it];
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + w);
+    const uint32_t l = lane % 8;
+    constexpr int U = 8;
+    uint32_t it;
     double acc = 0.0;
-    if (have) {
-        constexpr int U = 8;
+    if (wid < nwide) {
+        it = wid;
+        const K1Item d = items[it];
+        const uint32_t base = chp[d.c] + l, s0 = lane / 8;
+        for (uint32_t k = d.kbeg + s0; k < d.kend; k += 8 * U) {
+            double v[U];
+            uint32_t cc[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                uint32_t kk = k + 8 * j < d.kend ? k + 8 * j : d.kend - 1;
+                v[j] = ntload(val + base + 8 * kk);
+                cc[j] = ntload(col + base + 8 * kk);
+            }
+            double xv[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) xv[j] = x[cc[j]];
+#pragma unroll
+            for (int j = 0; j < U; j++)
+                if (k + 8 * j < d.kend) acc = fma(v[j], xv[j], acc);
+        }
+        acc += shfl_xor_f64(acc, 8);
+        acc += shfl_xor_f64(acc, 16);
+        acc += shfl_xor_f64(acc, 32);
+        if (lane >= 8) return;
+    } else {
+        it = nwide + (wid - nwide) * 8 + lane / 8;
+        if (it >= nitems) return;
+        const K1Item d = items[it];
         const uint32_t base = chp[d.c] + l;
         for (uint32_t k = d.kbeg; k < d.kend; k += U) {
             double v[U];
@@ -127,8 +160,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k1_kernel(
                 if (k + j < d.kend) acc = fma(v[j], xv[j], acc);
         }
     }
+    const K1Item d = items[it];
     const uint32_t s = d.c * 8 + l;
-    if (!have) return;
     if (d.k <= 1) {
         if (!permi) y[s] = acc;
         else if (s < nrows) y[permi[s]] = acc;
@@ -205,22 +238,25 @@ extern "C" int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkN
     if (!ctx || !plan || (chunkNum && !nmc_host)) return INTSPMV_ERR_ARG;
     if (chunkNum != (nrows + 7) / 8) return INTSPMV_ERR_ARG;
     *plan = nullptr;
+    // chunks are in length order (longest first): long chunks become wide
+    // items of at most K1_PIECE steps, the rest narrow items
     std::vector<K1Item> items;
     std::vector<SplitDesc> splits;
+    uint32_t nwide = 0;
     for (uint32_t c = 0; c < chunkNum; c++) {
         uint32_t n = nmc_host[c];
-        if (n <= K1_LONG) {
-            items.push_back({c, 0, n, (uint32_t)items.size(), 1, 0, 0, 0});
-            continue;
-        }
-        uint32_t np = (n + K1_LONG - 1) / K1_LONG, per = (n + np - 1) / np;
+        if (n <= K1_WIDE) continue;
+        uint32_t np = (n + K1_PIECE - 1) / K1_PIECE, per = (n + np - 1) / np;
         uint32_t b0 = (uint32_t)items.size();
         for (uint32_t q = 0; q < np; q++) {
             uint32_t a = q * per, b = a + per > n ? n : a + per;
             items.push_back({c, a, b, b0, np, 0, 0, 0});
         }
-        splits.push_back({c, b0, np, 0});
+        if (np > 1) splits.push_back({c, b0, np, 0});
     }
+    nwide = (uint32_t)items.size();
+    for (uint32_t c = 0; c < chunkNum; c++)
+        if (nmc_host[c] <= K1_WIDE) items.push_back({c, 0, nmc_host[c], (uint32_t)items.size(), 1, 0, 0, 0});
     intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));
     if (!p) return INTSPMV_ERR_NOMEM;
     p->ctx = ctx;
@@ -228,6 +264,7 @@ extern "C" int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkN
     p->nrows = nrows;
     p->chunkNum = chunkNum;
     p->nblocks = (uint32_t)items.size();
+    p->nlong_wg = nwide; // number of wide (one-item) waves
     p->nsplit_rows = (uint32_t)splits.size();
     int st = ispv_plan_upload(ctx, p, items.data(), sizeof(K1Item), items.size(), 8,
                               splits.data(), splits.size());
@@ -249,11 +286,12 @@ extern "C" int spmv_k1(intspmv_ctx *ctx, const intspmv_plan *plan, const double 
     if (st) return st;
     if (plan->nblocks == 0) return INTSPMV_OK;
     if (!chp || !x || !y || !val || !col) return INTSPMV_ERR_ARG;
-    uint32_t waves = (plan->nblocks + 7) / 8;
+    const uint32_t nwide = plan->nlong_wg;
+    uint32_t waves = nwide + (plan->nblocks - nwide + 7) / 8;
     dim3 grid((waves + WAVES - 1) / WAVES);
     k1_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(
-        (const K1Item *)plan->d_blocks, plan->nblocks, val, col, chp, permi, x, y, plan->nrows,
-        plan->d_partial);
+        (const K1Item *)plan->d_blocks, plan->nblocks, nwide, val, col, chp, permi, x, y,
+        plan->nrows, plan->d_partial);
     st = ispv_launch_check(ctx);
     if (st || plan->nsplit_rows == 0) return st;
     k1_fixup_kernel<<<(plan->nsplit_rows * 8 + 255) / 256, 256, 0, ctx->stream>>>(

@@ -108,6 +108,66 @@ __device__ __forceinline__ size_t xidx(size_t T, uint32_t th, uint32_t R, uint32
 
 // Long row of th==1 tiles [ubeg, uend): every lane keeps its own products'
 // running sum; one wave tree at the end.  UL steps of 16 tiles in flight.
+__device__ __forceinline__ float4 f4load(const float *p)
+{
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    f4v v = ntload(reinterpret_cast<const f4v *>(p));
+    return make_float4(v[0], v[1], v[2], v[3]);
+}
+template <int CTRL>
+__device__ __forceinline__ float4 f4dpp(float4 v)
+{
+    return make_float4(dpp_f32<CTRL>(v.x), dpp_f32<CTRL>(v.y), dpp_f32<CTRL>(v.z), dpp_f32<CTRL>(v.w));
+}
+// Sum of one planar tile (8 values, 8 x) read whole by one lane.
+template <typename VT, typename XT, typename AT>
+__device__ __forceinline__ AT planar_tile(const VT *av, const XT *xp, size_t T)
+{
+    AT v[8], x[8];
+    if constexpr (sizeof(VT) == 2) {
+        typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+        u4v b = ntload(reinterpret_cast<const u4v *>(av + T * 8));
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[2 * i] = __uint_as_float(b[i] << 16);
+            v[2 * i + 1] = __uint_as_float(b[i] & 0xFFFF0000u);
+        }
+    } else if constexpr (sizeof(VT) == 4) {
+        float4 a = f4load(reinterpret_cast<const float *>(av) + T * 8);
+        float4 b = f4load(reinterpret_cast<const float *>(av) + T * 8 + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            LoadT<VT> a, b;
+            Pair<VT>::ld(av + T * 8 + i, a, b);
+            v[i] = (AT)a;
+            v[i + 1] = (AT)b;
+        }
+    }
+    if constexpr (sizeof(XT) == 4) {
+        float4 a = f4load(reinterpret_cast<const float *>(xp) + T * 8);
+        float4 b = f4load(reinterpret_cast<const float *>(xp) + T * 8 + 4);
+        x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+        x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            LoadT<XT> a, b;
+            Pair<XT>::ld(xp + T * 8 + i, a, b);
+            x[i] = (AT)a;
+            x[i + 1] = (AT)b;
+        }
+    }
+    // same halving tree as the reference's reduce (any order is allowed for
+    // the reduced-precision storage types)
+    AT p[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) p[i] = v[i] * x[i];
+    return ((p[0] + p[4]) + (p[2] + p[6])) + ((p[1] + p[5]) + (p[3] + p[7]));
+}
+
 #ifndef ISPV_H1_LONG_STEPS
 #define ISPV_H1_LONG_STEPS 8
 #endif
@@ -135,6 +195,45 @@ __device__ __forceinline__ AT h1_long(const VT *av, const XT *xb, const BlockDes
             }
         }
         return (AT)wave_sum_f64(acc);
+    }
+    if constexpr (!PLANAR && sizeof(VT) == 4) {
+        // FP32 interleaved: a 128-byte line holds two 64-byte tiles; part
+        // p = l%8 of line l/8: values (p%4 < 2) or x (p%4 >= 2), 4 each
+        const uint32_t nu = d.uend - d.ubeg, p = lane & 7;
+        LaneSum<float> acc;
+        for (uint32_t base = 0; base < nu; base += 16 * UL) {
+            float4 v[UL];
+#pragma unroll
+            for (int j = 0; j < UL; j++) {
+                uint32_t t = base + j * 16 + 2 * (lane / 8) + p / 4;
+                size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+                v[j] = f4load(reinterpret_cast<const float *>(av) + T * 16 + 4 * (p % 4));
+            }
+#pragma unroll
+            for (int j = 0; j < UL; j++) {
+                float4 xx = f4dpp<DPP_ROW_SHL2>(v[j]);
+                if (base + j * 16 + 2 * (lane / 8) + p / 4 < nu && p % 4 < 2)
+                    acc.add(v[j].x * xx.x + v[j].y * xx.y + (v[j].z * xx.z + v[j].w * xx.w));
+            }
+        }
+        return (AT)wave_sum_f32(acc.value());
+    }
+    if constexpr (PLANAR) {
+        // planar: one lane per tile, the tile's 8 values and 8 x whole
+        const uint32_t nu = d.uend - d.ubeg;
+        LaneSum<AT> acc;
+        for (uint32_t base = 0; base < nu; base += 64 * UL / 2) {
+            AT ts[UL / 2];
+#pragma unroll
+            for (int j = 0; j < UL / 2; j++) {
+                uint32_t t = base + j * 64 + lane;
+                ts[j] = planar_tile<VT, XT, AT>(av, xb, (size_t)d.ubeg + (t < nu ? t : nu - 1));
+            }
+#pragma unroll
+            for (int j = 0; j < UL / 2; j++)
+                if (base + j * 64 + lane < nu) acc.add(ts[j]);
+        }
+        return wave_sum<AT>(acc.value());
     }
     const uint32_t nu = d.uend - d.ubeg, C = 2 * (lane & 3);
     LaneSum<AT> acc;
@@ -211,6 +310,62 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
                 uint32_t t = base + j * 8 + lane / 8;
                 if (t < nu && p == 0) {
                     s_val[w][t] = (AT)(p0 + p1);
+                    s_key[w][t] = key[j];
+                }
+            }
+        }
+        wave_lds_sync();
+        reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
+        return;
+    }
+    if constexpr (!PLANAR && sizeof(VT) == 4) {
+        // FP32 interleaved, two 64-byte tiles per 128-byte line (see h1_long)
+        const uint32_t p = lane & 7;
+        constexpr int U = 2 * ISPV_H1_STEPS;
+        for (uint32_t base = 0; base < nu; base += 16 * U) {
+            float4 v[U];
+            uint32_t key[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                uint32_t t = base + j * 16 + 2 * (lane / 8) + p / 4;
+                size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+                v[j] = f4load(reinterpret_cast<const float *>(av) + T * 16 + 4 * (p % 4));
+                key[j] = ntload(sec + T);
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                uint32_t t = base + j * 16 + 2 * (lane / 8) + p / 4;
+                float4 xx = f4dpp<DPP_ROW_SHL2>(v[j]);
+                float ps = (v[j].x * xx.x + v[j].y * xx.y) + (v[j].z * xx.z + v[j].w * xx.w);
+                ps = ps + dpp_f32<DPP_QUAD_XOR1>(ps); // values 0-3 + values 4-7
+                if (t < nu && p % 4 == 0) {
+                    s_val[w][t] = (AT)ps;
+                    s_key[w][t] = key[j];
+                }
+            }
+        }
+        wave_lds_sync();
+        reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
+        return;
+    }
+    if constexpr (PLANAR) {
+        // planar storage: one lane per tile
+        constexpr int U = ISPV_H1_STEPS;
+        for (uint32_t base = 0; base < nu; base += 64 * U) {
+            AT ts[U];
+            uint32_t key[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                uint32_t t = base + j * 64 + lane;
+                size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+                ts[j] = planar_tile<VT, XT, AT>(av, xb, T);
+                key[j] = ntload(sec + T);
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                uint32_t t = base + j * 64 + lane;
+                if (t < nu) {
+                    s_val[w][t] = ts[j];
                     s_key[w][t] = key[j];
                 }
             }

@@ -15,9 +15,15 @@
 
 namespace ispv {
 
-constexpr uint32_t CSR_UMAX = 512;
+#ifndef ISPV_CSR_UMAX
+#define ISPV_CSR_UMAX 512
+#endif
+#ifndef ISPV_CSR_LONG
+#define ISPV_CSR_LONG 256
+#endif
+constexpr uint32_t CSR_UMAX = ISPV_CSR_UMAX;
 constexpr uint32_t CSR_RMAX = 256;
-constexpr uint32_t CSR_LONG = 256;
+constexpr uint32_t CSR_LONG = ISPV_CSR_LONG;
 constexpr uint32_t CSR_LONG_MAX = 8192; // longest row one wave streams alone
 
 __global__ __launch_bounds__(BLOCK_THREADS) void csr_ordered_kernel(
@@ -65,6 +71,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
         finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial);
         return;
     }
+    // this lane's first row and its bounds, fetched before the stream
+    const uint32_t r0 = d.rbeg + lane;
+    uint32_t ra = 0, rb = 0;
+    if (r0 < d.rend) {
+        ra = row[r0];
+        rb = row[r0 + 1];
+    }
     for (uint32_t base = 0; base < nu; base += WAVE * U) {
         double v[U];
         uint32_t c[U];
@@ -88,8 +101,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
         }
     }
     wave_lds_sync();
-    for (uint32_t r = d.rbeg + lane; r < d.rend; r += WAVE) {
-        uint32_t a = row[r] - d.ubeg, b = row[r + 1] - d.ubeg;
+    for (uint32_t r = r0; r < d.rend; r += WAVE) {
+        uint32_t a = (r == r0 ? ra : row[r]) - d.ubeg, b = (r == r0 ? rb : row[r + 1]) - d.ubeg;
         double acc = 0.0;
         for (uint32_t i = a; i < b; i++) acc = fma(s_v[w][i], s_x[w][i], acc);
         y[r] = acc;
