@@ -11,7 +11,7 @@
  *     intspmv_strerror() names it.  Nothing calls exit().
  *   - Launches are asynchronous on the context's stream (intspmv_set_stream).
  *     One context per host thread; a plan must not be used by two launches
- *     that can overlap in time (its split-row counters are shared).
+ *     that can overlap in time (its split-row scratch is shared).
  *   - y semantics: all spmv_* entry points OVERWRITE y (y = A*x), including
  *     rows with no entries.  The reference's AXT kernels instead accumulate
  *     into a y their caller zeroed (intSpmv_main.c:1421, 1492, 1591); a host
@@ -131,7 +131,44 @@ int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc, const uint3
 int spmv_axt(intspmv_ctx *ctx, const intspmv_plan *plan, const void *ax, const void *xplane,
              const uint32_t *sec, void *y);
 
+/* ---- AXT with a column map (x not embedded) -----------------------------
+ * aval[s] / acol[s]: value and column of slot s in the tile order above
+ * (acol = 0xFFFFFFFF on padding), sec as built.  The plan is the AXT plan of
+ * the same matrix (its val_type / x_type select the kernel; layout is not
+ * used).  X is n x k row-major with leading dimension ldx >= k, Y likewise;
+ * 1 <= k <= 32 right-hand sides.  Y = A X.  No reference counterpart: the
+ * reference embeds x (intSpmv_main.c:1171, 1207), which fixes one x per build. */
+int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval,
+                    const uint32_t *acol, const uint32_t *sec, const void *x, uint32_t ldx,
+                    void *y, uint32_t ldy, uint32_t k);
+/* Value gradient on the pattern: vbar[s] = ybar[row(s)] * x[acol[s]] (0 on
+ * padding), one entry per slot (lenAX/2).  F64 or F32. */
+int spmv_axt_grad_vals(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_t *acol,
+                       const uint32_t *sec, const void *x, const void *ybar, void *vbar);
+/* Same for the reference layout, whose tiles already hold x[col]:
+ * vbar[s] = ybar[row(s)] * (x half of slot s). */
+int spmv_axt_grad_vals_embedded(intspmv_ctx *ctx, const intspmv_plan *plan, const double *ax,
+                                const uint32_t *sec, const double *ybar, double *vbar);
+/* dst[s] = src[perm[s]] (0 where perm[s] == 0xFFFFFFFF) for s < n: moves slot
+ * values between the slot orders of A and of A^T (the transposed map). */
+int spmv_permute_vals(intspmv_ctx *ctx, uint32_t n, int val_type, const uint32_t *perm,
+                      const void *src, void *dst);
+/* Device-side conversion: rewrites the embedded x of every occupied slot of
+ * the reference layout from acol and a new x (the host builder's ax[..+8] =
+ * vec[col] step, intSpmv_main.c:1171, 1207), so x can change without a rebuild. */
+int spmv_axt_refresh_x(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_t *acol,
+                       const double *x, double *ax);
+
 int intspmv_plan_destroy(intspmv_plan *plan);
+/* Host-only inspection of the work decomposition a plan uses for a row ->
+ * unit pointer uptr[nrows+1]: blocks (8 words each: ubeg, uend, rbeg, rend,
+ * b0, k, piece, 0) and split rows (4 words: row, first piece, pieces, 0).
+ * Arrays are filled when their capacity (in entries) suffices; the counts are
+ * always returned.  No device is touched. */
+int intspmv_plan_layout(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
+                        uint32_t long_units, uint32_t long_max, uint32_t *blocks,
+                        size_t cap_blocks, uint32_t *nblocks, uint32_t *splits,
+                        size_t cap_splits, uint32_t *nsplits, uint32_t *nlong_wg);
 /* number of work blocks / split rows of a plan (for reporting) */
 int intspmv_plan_info(const intspmv_plan *plan, uint32_t *nblocks, uint32_t *nsplit_rows,
                       uint32_t *nlong_rows);

@@ -92,6 +92,7 @@ typedef struct {
     uint32_t bs;   /* COM block size (0 for NOC) */
     uint32_t colN; /* occupied columns (th>1) / slots (th==1) */
     uint32_t *acol; /* [lenAX/2] or NULL when not requested */
+    uint32_t *nzslot; /* [nnz]: slot of CSR entry k (with acol), else NULL */
 } intspmv_axt;
 
 /* ---- matrix I/O (intSpmv_main.c:187-249, plus Matrix Market) ---------- */
@@ -105,6 +106,10 @@ int intspmv_csr_write_mtx(const char *path, const intspmv_csr *m);
 /* Allocates val/col/row/rl for a matrix of the given size (zeroed). */
 int intspmv_csr_alloc(intspmv_csr *m, uint32_t nrows, uint32_t nnz);
 void intspmv_csr_free(intspmv_csr *m);
+/* Transpose: at = A^T in CSR form (rows of at = columns of A, entries of a
+ * row in ascending column order); perm[k'] = index in A of entry k' of at
+ * (perm may be NULL). */
+int intspmv_csr_transpose(const intspmv_csr *a, intspmv_csr *at, uint32_t *perm);
 /* Builds CSR from COO triplets: sorts by (row, col), sums duplicates. */
 int intspmv_csr_from_coo(uint32_t nrows, size_t n_entries, const uint32_t *ri,
                          const uint32_t *ci, const double *v, intspmv_csr *out);
@@ -125,7 +130,7 @@ int intspmv_k1_build(const intspmv_csr *m, intspmv_k1 *out, intspmv_format_data 
 void intspmv_k1_free(intspmv_k1 *k);
 
 /* mode "NOC" (th >= 1) or "COM" (th == 1, bs a power of two in [2, 512]);
- * thw must be 8.  want_acol != 0 also fills acol. */
+ * thw must be 8.  want_acol != 0 also fills acol and nzslot. */
 int intspmv_axt_build(const intspmv_csr *m, const double *x, uint32_t bs, uint32_t thw,
                       uint32_t th, const char *mode, int want_acol, intspmv_axt *out,
                       intspmv_format_data *fd);

@@ -1,0 +1,59 @@
+"""torch.autograd.Function over the C ABI (used by the test harness).
+
+y = A(vals) x with A an AXT NOC matrix in column-map form.  Backward:
+  x_bar    = A^T y_bar   -- spmv_axt_gather on the transposed tile map
+  vals_bar = y_bar[row(s)] * x[col(s)] per slot -- spmv_axt_grad_vals
+Both are hand-written gfx950 kernels; torch only holds the tensors.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import device as D
+
+
+class AxtOperator:
+    """Structure of A (slot order, column map, plan) and of A^T."""
+
+    def __init__(self, eng: D.Engine, m, th: int = 1, val_type: int = D.F64):
+        from . import host as H
+        self.eng, self.m = eng, m
+        zero = __import__("numpy").zeros(m.nrows)
+        self.a = H.AXT(m, zero, th=th, want_acol=True)
+        self.fwd = D.DeviceAXTGather(eng, self.a, val_type, val_type)
+        self.adj = D.AxtAdjoint(eng, m, self.fwd)
+        self.dtype = torch.float64 if val_type == D.F64 else torch.float32
+
+    def initial_values(self):
+        """The matrix values in slot order (a leaf tensor to optimise)."""
+        return self.fwd.aval.clone()
+
+
+class AxtSpMV(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vals, x, op: AxtOperator):
+        vals, x = vals.contiguous(), x.contiguous()
+        y = torch.empty(op.m.nrows, dtype=op.dtype, device=x.device)
+        op.fwd.matvec(x, y, vals=vals)
+        ctx.save_for_backward(vals, x)
+        ctx.op = op
+        return y
+
+    @staticmethod
+    def backward(ctx, ybar):
+        vals, x = ctx.saved_tensors
+        op = ctx.op
+        ybar = ybar.contiguous()
+        gx = gv = None
+        if ctx.needs_input_grad[1]:
+            op.adj.refresh(vals)
+            gx = torch.empty_like(x)
+            op.adj.adjoint(ybar, gx)
+        if ctx.needs_input_grad[0]:
+            gv = torch.empty_like(vals)
+            op.fwd.grad_vals(x, ybar, gv)
+        return gv, gx, None
+
+
+def spmv(vals, x, op: AxtOperator):
+    return AxtSpMV.apply(vals, x, op)

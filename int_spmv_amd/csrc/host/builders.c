@@ -247,7 +247,10 @@ static void axt_noc_h1(const intspmv_csr *m, const double *x, intspmv_axt *a)
             a->ax[pos] = m->val[k];
             a->ax[pos + thw] = x[m->col[k]];
             a->sec[psec] = r;
-            if (a->acol) a->acol[axt_slot(pos, thw, ths)] = m->col[k];
+            if (a->acol) {
+                a->acol[axt_slot(pos, thw, ths)] = m->col[k];
+                a->nzslot[k] = axt_slot(pos, thw, ths);
+            }
             pos++;
             cnt++;
             if (cnt % thw == 0) {
@@ -271,7 +274,10 @@ static void axt_noc(const intspmv_csr *m, const double *x, intspmv_axt *a)
             a->ax[pos] = m->val[k];
             a->ax[pos + thw] = x[m->col[k]];
             a->sec[psec] = r;
-            if (a->acol) a->acol[axt_slot(pos, thw, ths)] = m->col[k];
+            if (a->acol) {
+                a->acol[axt_slot(pos, thw, ths)] = m->col[k];
+                a->nzslot[k] = axt_slot(pos, thw, ths);
+            }
             pos += 2 * thw;
             cnt++;
             if (cnt % th == 0) {
@@ -295,7 +301,10 @@ static void axt_com_h1(const intspmv_csr *m, const double *x, intspmv_axt *a)
             uint32_t pos = tid * 2 * thw + tco;
             a->ax[pos] = m->val[k];
             a->ax[pos + thw] = x[m->col[k]];
-            if (a->acol) a->acol[k] = m->col[k];
+            if (a->acol) {
+                a->acol[k] = m->col[k];
+                a->nzslot[k] = k;
+            }
             if (e == 0 || bco == 0) {
                 uint32_t q1 = rl - e - 1, q2 = bs - 1 - bco;
                 uint32_t off = q1 > q2 ? q2 : q1;
@@ -349,10 +358,11 @@ int intspmv_axt_build(const intspmv_csr *m, const double *x, uint32_t bs, uint32
     if (want_acol) {
         size_t ns = (size_t)out->lenAX / 2;
         out->acol = (uint32_t *)malloc((ns ? ns : 1) * sizeof(uint32_t));
+        out->nzslot = (uint32_t *)malloc((m->nnz ? m->nnz : 1) * sizeof(uint32_t));
         if (out->acol)
             for (size_t s = 0; s < ns; s++) out->acol[s] = INTSPMV_ACOL_PAD;
     }
-    if (!out->ax || !out->sec || (want_acol && !out->acol)) {
+    if (!out->ax || !out->sec || (want_acol && (!out->acol || !out->nzslot))) {
         intspmv_axt_free(out);
         return INTSPMV_HOST_ENOMEM;
     }
@@ -400,5 +410,6 @@ void intspmv_axt_free(intspmv_axt *a)
     free(a->sec);
     free(a->con);
     free(a->acol);
+    free(a->nzslot);
     memset(a, 0, sizeof(*a));
 }

@@ -189,6 +189,30 @@ int intspmv_csr_from_coo(uint32_t nrows, size_t n_entries, const uint32_t *ri,
     return INTSPMV_HOST_OK;
 }
 
+int intspmv_csr_transpose(const intspmv_csr *a, intspmv_csr *at, uint32_t *perm)
+{
+    if (!a || !at) return INTSPMV_HOST_EARG;
+    int st = intspmv_csr_alloc(at, a->nrows, a->nnz);
+    if (st) return st;
+    for (uint32_t k = 0; k < a->nnz; k++) at->row[a->col[k] + 1]++;
+    for (uint32_t i = 0; i < a->nrows; i++) at->row[i + 1] += at->row[i];
+    uint32_t *next = (uint32_t *)malloc(((size_t)a->nrows + 1) * sizeof(uint32_t));
+    if (!next) {
+        intspmv_csr_free(at);
+        return INTSPMV_HOST_ENOMEM;
+    }
+    memcpy(next, at->row, ((size_t)a->nrows + 1) * sizeof(uint32_t));
+    for (uint32_t i = 0; i < a->nrows; i++)
+        for (uint32_t k = a->row[i]; k < a->row[i + 1]; k++) {
+            uint32_t q = next[a->col[k]]++;
+            at->col[q] = i;
+            at->val[q] = a->val[k];
+            if (perm) perm[q] = k;
+        }
+    free(next);
+    return intspmv_csr_stats(at, NULL);
+}
+
 static int read_mtx(const char *path, intspmv_csr *out)
 {
     FILE *fh = fopen(path, "r");

@@ -162,31 +162,33 @@ extern "C" int intspmv_memcpy_dtoh(intspmv_ctx *ctx, void *dst, const void *src,
     return copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost);
 }
 
-// Split-row fix-up: one thread per split row adds the row's piece partials
-// in piece order (deterministic) and stores the result.
+// Split-row fix-up: one thread per (split row, right-hand side) adds the
+// row's piece partials in piece order (deterministic) and stores the result
+// at y[row * ldy + j0 + j].
 template <typename YT>
 __global__ void split_fixup_kernel(const ispv::SplitDesc *__restrict__ splits, uint32_t nsplit,
                                    const double *__restrict__ partial, YT *__restrict__ y,
-                                   uint32_t nrows)
+                                   uint32_t nrows, uint32_t k, uint32_t ldy, uint32_t j0)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, i = t / k, j = t % k;
     if (i >= nsplit) return;
     const ispv::SplitDesc s = splits[i];
-    double t = 0.0;
-    for (uint32_t j = 0; j < s.k; j++) t = t + partial[s.b0 + j];
-    if (s.row < nrows) y[s.row] = (YT)t;
+    double acc = 0.0;
+    for (uint32_t q = 0; q < s.k; q++) acc = acc + partial[(size_t)(s.b0 + q) * ispv::PARTIAL_STRIDE + j];
+    if (s.row < nrows) y[(size_t)s.row * ldy + j0 + j] = (YT)acc;
 }
 
-int ispv_split_fixup(const intspmv_plan *p, void *y, int y_is_f32)
+int ispv_split_fixup(const intspmv_plan *p, void *y, int y_is_f32, uint32_t k, uint32_t ldy,
+                     uint32_t j0)
 {
     if (p->nsplit_rows == 0) return INTSPMV_OK;
-    dim3 grid((p->nsplit_rows + 255) / 256);
+    dim3 grid((p->nsplit_rows * k + 255) / 256);
     if (y_is_f32)
         split_fixup_kernel<float><<<grid, 256, 0, p->ctx->stream>>>(
-            p->d_splits, p->nsplit_rows, p->d_partial, (float *)y, p->nrows);
+            p->d_splits, p->nsplit_rows, p->d_partial, (float *)y, p->nrows, k, ldy, j0);
     else
         split_fixup_kernel<double><<<grid, 256, 0, p->ctx->stream>>>(
-            p->d_splits, p->nsplit_rows, p->d_partial, (double *)y, p->nrows);
+            p->d_splits, p->nsplit_rows, p->d_partial, (double *)y, p->nrows, k, ldy, j0);
     return ispv_launch_check(p->ctx);
 }
 
@@ -199,7 +201,9 @@ int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size
     if (st) return st;
     size_t nb = nblocks ? nblocks : 1, ns = nsplit ? nsplit : 1;
     hipError_t e = hipMalloc(&p->d_blocks, nb * block_bytes);
-    if (e == hipSuccess) e = hipMalloc(&p->d_partial, nb * partial_per_block * sizeof(double));
+    size_t npart = partial_per_block ? nb * partial_per_block
+                                     : (size_t)(p->npieces ? p->npieces : 1) * ispv::PARTIAL_STRIDE;
+    if (e == hipSuccess) e = hipMalloc(&p->d_partial, npart * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&p->d_splits, ns * sizeof(ispv::SplitDesc));
     if (e == hipSuccess && nblocks)
         e = hipMemcpy(p->d_blocks, blocks, nblocks * block_bytes, hipMemcpyHostToDevice);
@@ -220,19 +224,19 @@ int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size
 // long blocks of one wave each; rows longer than long_max are split into
 // ceil(len / long_max) near-equal pieces combined by the fix-up kernel.  The
 // long blocks come first, longest first, padded to whole XCD rounds of
-// workgroups; the normal blocks follow in row order.
-int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
-                    uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_max,
-                    intspmv_plan **out)
+// workgroups; the normal blocks follow in row order.  Pure host code.
+static int build_blocks(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
+                        uint32_t long_units, uint32_t long_max,
+                        std::vector<ispv::BlockDesc> &blocks,
+                        std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg,
+                        uint32_t &nlong, uint32_t &npieces)
 {
-    if (!out || !uptr || umax == 0 || rmax == 0 || long_units >= umax || long_max <= long_units)
+    if (!uptr || umax == 0 || rmax == 0 || long_units >= umax || long_max <= long_units)
         return INTSPMV_ERR_ARG;
-    *out = nullptr;
     for (uint32_t r = 0; r < nrows; r++)
         if (uptr[r + 1] < uptr[r]) return INTSPMV_ERR_ARG;
     std::vector<ispv::BlockDesc> normal, lng;
-    std::vector<uint32_t> split_rows;
-    uint32_t nlong = 0;
+    nlong = 0;
     uint32_t r = 0;
     while (r < nrows) {
         uint32_t len = uptr[r + 1] - uptr[r];
@@ -246,10 +250,10 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
                 for (uint32_t q = 0; q < np; q++) {
                     uint64_t a = uptr[r] + q * per, b = a + per;
                     if (b > uptr[r + 1]) b = uptr[r + 1];
-                    // b0 is fixed up below once pieces have their final index
-                    lng.push_back({(uint32_t)a, (uint32_t)b, r, r + 1, q, np, 0, 0});
+                    // pad1 = index of the piece within its row; b0 and piece
+                    // are filled in below once the pieces have their final place
+                    lng.push_back({(uint32_t)a, (uint32_t)b, r, r + 1, 0, np, 0, q});
                 }
-                split_rows.push_back(r);
             }
             r++;
             continue;
@@ -269,16 +273,64 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
         uint32_t lb = b.k > 1 ? 0xFFFFFFFFu - b.rbeg : b.uend - b.ubeg;
         return la > lb;
     });
-    std::vector<ispv::SplitDesc> splits;
+    splits.clear();
+    npieces = 0;
     for (size_t i = 0; i < lng.size(); i++)
-        if (lng[i].k > 1 && lng[i].b0 == 0) splits.push_back({lng[i].rbeg, (uint32_t)i, lng[i].k, 0});
-    for (auto &sp : splits)
-        for (uint32_t q = 0; q < sp.k; q++) lng[sp.b0 + q].b0 = sp.b0;
+        if (lng[i].k > 1 && lng[i].pad1 == 0) {
+            // the k pieces of this row follow in order (stable sort, equal keys)
+            const uint32_t k = lng[i].k;
+            if (i + k > lng.size()) return INTSPMV_ERR_ARG;
+            splits.push_back({lng[i].rbeg, npieces, k, 0});
+            for (uint32_t q = 0; q < k; q++) {
+                lng[i + q].b0 = (uint32_t)i;
+                lng[i + q].piece = npieces++;
+                lng[i + q].pad1 = 0;
+            }
+            i += k - 1;
+        }
     const uint32_t per_round = ispv::WAVES * ispv::NUM_XCD;
     size_t nl = lng.empty() ? 0 : (lng.size() + per_round - 1) / per_round * per_round;
     while (lng.size() < nl) lng.push_back({0, 0, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 1, 0, 0});
-    std::vector<ispv::BlockDesc> blocks(lng);
+    blocks = lng;
     blocks.insert(blocks.end(), normal.begin(), normal.end());
+    nlong_wg = (uint32_t)(nl / ispv::WAVES);
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_plan_layout(uint32_t nrows, const uint32_t *uptr, uint32_t umax,
+                                   uint32_t rmax, uint32_t long_units, uint32_t long_max,
+                                   uint32_t *blocks, size_t cap_blocks, uint32_t *nblocks,
+                                   uint32_t *splits, size_t cap_splits, uint32_t *nsplits,
+                                   uint32_t *nlong_wg)
+{
+    std::vector<ispv::BlockDesc> b;
+    std::vector<ispv::SplitDesc> sp;
+    uint32_t nlw = 0, nlong = 0, npieces = 0;
+    int st = build_blocks(nrows, uptr, umax, rmax, long_units, long_max, b, sp, nlw, nlong,
+                          npieces);
+    if (st) return st;
+    if (nblocks) *nblocks = (uint32_t)b.size();
+    if (nsplits) *nsplits = (uint32_t)sp.size();
+    if (nlong_wg) *nlong_wg = nlw;
+    if (blocks && cap_blocks >= b.size())
+        memcpy(blocks, b.data(), b.size() * sizeof(ispv::BlockDesc));
+    if (splits && cap_splits >= sp.size())
+        memcpy(splits, sp.data(), sp.size() * sizeof(ispv::SplitDesc));
+    return INTSPMV_OK;
+}
+
+int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
+                    uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_max,
+                    intspmv_plan **out)
+{
+    if (!out) return INTSPMV_ERR_ARG;
+    *out = nullptr;
+    std::vector<ispv::BlockDesc> blocks;
+    std::vector<ispv::SplitDesc> splits;
+    uint32_t nlw = 0, nlong = 0, npieces = 0;
+    int st = build_blocks(nrows, uptr, umax, rmax, long_units, long_max, blocks, splits, nlw,
+                          nlong, npieces);
+    if (st) return st;
     intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));
     if (!p) return INTSPMV_ERR_NOMEM;
     p->ctx = ctx;
@@ -286,14 +338,15 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
     p->nrows = nrows;
     p->nunits = nrows ? uptr[nrows] : 0;
     p->nblocks = (uint32_t)blocks.size();
-    p->nlong_wg = (uint32_t)(nl / ispv::WAVES);
+    p->nlong_wg = nlw;
     p->nsplit_rows = (uint32_t)splits.size();
     p->nlong_rows = nlong;
     p->umax = umax;
     p->rmax = rmax;
     p->long_units = long_units;
-    int st = ispv_plan_upload(ctx, p, blocks.data(), sizeof(ispv::BlockDesc), blocks.size(), 1,
-                              splits.data(), splits.size());
+    p->npieces = npieces;
+    st = ispv_plan_upload(ctx, p, blocks.data(), sizeof(ispv::BlockDesc), blocks.size(), 0,
+                          splits.data(), splits.size());
     if (st) {
         intspmv_plan_destroy(p);
         return st;

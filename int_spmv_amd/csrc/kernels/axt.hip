@@ -22,9 +22,6 @@
 
 namespace ispv {
 
-#ifndef ISPV_AXT_UMAX
-#define ISPV_AXT_UMAX 128
-#endif
 #ifndef ISPV_H1_STEPS
 #define ISPV_H1_STEPS 4
 #endif

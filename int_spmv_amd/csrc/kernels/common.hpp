@@ -19,6 +19,12 @@ constexpr int WAVES = 4;            // waves per workgroup
 constexpr int BLOCK_THREADS = WAVE * WAVES;
 constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin
 
+// AXT units (tiles for th == 1, tile columns for th > 1) per normal block;
+// the embedded and column-map kernels share one plan, so one constant.
+#ifndef ISPV_AXT_UMAX
+#define ISPV_AXT_UMAX 128
+#endif
+
 // One unit of work for one wave.  Units are the format's reduction atoms
 // (CSR entries, AXC bricks, AXT tiles for th==1, AXT tile columns for th>1,
 // AXT COM slots).  Rows [rbeg, rend) own units [ubeg, uend).
@@ -28,10 +34,15 @@ constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, workgroups dispatched rou
 //   k >= 2 : piece of a long row split over k consecutive blocks b0..b0+k-1;
 //            each piece leaves its partial sum in the plan's scratch and the
 //            split fix-up kernel adds them in piece order.
+//   piece   (k >= 2): index of this piece among all split pieces of the plan
+//            (its slot in the partial-sum scratch).
 struct BlockDesc {
     uint32_t ubeg, uend, rbeg, rend;
-    uint32_t b0, k, pad0, pad1;
+    uint32_t b0, k, piece, pad1;
 };
+// Partial sums of split pieces: PARTIAL_STRIDE slots per piece, one per
+// right-hand side (multi-RHS products use up to 32).
+constexpr uint32_t PARTIAL_STRIDE = 32;
 
 // Maps the flat workgroup id so that XCD x (which receives ids x, x+8, ...)
 // gets a contiguous range [x*q + min(x, r), ...) of logical workgroups.
@@ -128,6 +139,7 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_floa
 // writes its partial sum to partial[block] and a second, small kernel adds
 // the k partials in piece order once the main kernel has finished (the kernel
 // boundary orders the writes; no device-wide fences in the streaming kernel).
+// b0: piece index of the row's first piece (pieces are consecutive).
 struct SplitDesc {
     uint32_t row, b0, k, pad;
 };
@@ -150,7 +162,8 @@ struct intspmv_plan {
     uint32_t nlong_wg;  // workgroups of the long-row section (multiple of 8)
     uint32_t umax, rmax, long_units;
     ispv::BlockDesc *d_blocks;
-    double *d_partial;          // [nblocks] (K1: [nitems * 8])
+    double *d_partial;          // [npieces * PARTIAL_STRIDE] (K1: [nitems * 8])
+    uint32_t npieces;
     ispv::SplitDesc *d_splits;  // [nsplit_rows]
     uint32_t chunkNum;  // K1
     intspmv_axt_desc axt;
@@ -162,7 +175,8 @@ int ispv_launch_check(intspmv_ctx *ctx);
 int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size_t block_bytes,
                      size_t nblocks, size_t partial_per_block, const ispv::SplitDesc *splits,
                      size_t nsplit);
-int ispv_split_fixup(const intspmv_plan *p, void *y, int y_is_f32);
+int ispv_split_fixup(const intspmv_plan *p, void *y, int y_is_f32, uint32_t k = 1,
+                     uint32_t ldy = 1, uint32_t j0 = 0);
 int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
                     uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_max,
                     intspmv_plan **out);

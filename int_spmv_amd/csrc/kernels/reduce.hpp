@@ -40,11 +40,12 @@ template <typename AT>
 __device__ __forceinline__ void finish_long(AT total, const BlockDesc &d, uint32_t bid, AT *y,
                                             uint32_t nrows, double *partial)
 {
+    (void)bid;
     if (lane_id() != 0) return;
     if (d.k == 1) {
         if (d.rbeg < nrows) y[d.rbeg] = total;
     } else {
-        partial[bid] = (double)total;
+        partial[(size_t)d.piece * PARTIAL_STRIDE] = (double)total;
     }
 }
 

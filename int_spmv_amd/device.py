@@ -70,9 +70,34 @@ SIGNATURES = [
     ("spmv_k1_order", C.c_int, [_vp, _u32, _vp, _vp, _vp]),
     ("intspmv_axt_plan", C.c_int, [_vp, C.POINTER(AxtDesc), _vp, C.POINTER(_vp)]),
     ("spmv_axt", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("spmv_axt_gather", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32]),
+    ("spmv_axt_grad_vals", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("spmv_axt_grad_vals_embedded", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("spmv_permute_vals", C.c_int, [_vp, _u32, C.c_int, _vp, _vp, _vp]),
+    ("spmv_axt_refresh_x", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("intspmv_plan_destroy", C.c_int, [_vp]),
     ("intspmv_plan_info", C.c_int, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
+    ("intspmv_plan_layout", C.c_int, [_u32, _vp, _u32, _u32, _u32, _u32, _vp, C.c_size_t,
+                                      C.POINTER(_u32), _vp, C.c_size_t, C.POINTER(_u32),
+                                      C.POINTER(_u32)]),
 ]
+
+
+def plan_layout(uptr: np.ndarray, umax: int, rmax: int, long_units: int, long_max: int):
+    """Host-only: (blocks[nb, 8], splits[ns, 4], nlong_wg) of a block plan."""
+    uptr = np.ascontiguousarray(uptr, dtype=np.uint32)
+    nb, ns, nl = _u32(), _u32(), _u32()
+    n = len(uptr) - 1
+    st = lib().intspmv_plan_layout(n, _np_ptr(uptr), umax, rmax, long_units, long_max, None, 0,
+                                   C.byref(nb), None, 0, C.byref(ns), C.byref(nl))
+    if st:
+        raise IntSpmvError(st, "plan_layout")
+    b = np.zeros((max(nb.value, 1), 8), np.uint32)
+    s = np.zeros((max(ns.value, 1), 4), np.uint32)
+    lib().intspmv_plan_layout(n, _np_ptr(uptr), umax, rmax, long_units, long_max, _np_ptr(b),
+                              nb.value, C.byref(nb), _np_ptr(s), ns.value, C.byref(ns),
+                              C.byref(nl))
+    return b[: nb.value], s[: ns.value], nl.value
 
 _lib = None
 
@@ -300,3 +325,106 @@ class DeviceAXT:
     def int_axt(self, y) -> None:
         self.eng._check(lib().spmv_axt(self.eng.h, self.plan.h, _dptr(self.ax), _dptr(self.xplane),
                                        _dptr(self.sec), _dptr(y)), "spmv_axt")
+
+
+PAD = 0xFFFFFFFF
+
+
+def _value_plane(a, val_type: int) -> np.ndarray:
+    vp = a.value_plane()
+    return {F64: vp, F32: vp.astype(np.float32), BF16: bf16_bits(vp)}[val_type]
+
+
+def _axt_desc(a, val_type, x_type, layout) -> AxtDesc:
+    return AxtDesc(nrows=a.nrows, nnz=a.nnz, tileHW=a.tileHW, tileH=a.tileH, tileN=a.tileN,
+                   lenAX=a.lenAX, lenSEC=a.lenSEC, colN=a.colN, bs=a.bs, log=a.log,
+                   mode=NOC if a.mode == "NOC" else COM, val_type=val_type, x_type=x_type,
+                   layout=layout)
+
+
+class DeviceAXTGather:
+    """AXT NOC with a column map: values and columns in slot order, x separate
+    (n x k row-major, 1 <= k <= 32).  ``a`` must be built with want_acol."""
+
+    def __init__(self, eng: Engine, a, val_type: int = F64, x_type: int = F64, vals=None):
+        if a.mode != "NOC":
+            raise ValueError("the column-map kernels take NOC layouts")
+        self.eng, self.nrows, self.a = eng, a.nrows, a
+        self.val_type, self.x_type = val_type, x_type
+        self.nslots = a.lenAX // 2
+        self.desc = _axt_desc(a, val_type, x_type, PLANAR)
+        self.aval = vals if vals is not None else eng.upload(_value_plane(a, val_type))
+        self.acol = eng.upload(a.acol)
+        self.sec = eng.upload(a.sec)
+        h = _vp()
+        con = np.ascontiguousarray(a.con)
+        eng._check(lib().intspmv_axt_plan(eng.h, C.byref(self.desc), _np_ptr(con), C.byref(h)),
+                   "axt_plan")
+        self.plan = _Plan(eng, h)
+
+    def matvec(self, x, y, k: int = 1, ldx: int | None = None, ldy: int | None = None,
+               vals=None) -> None:
+        """y = A x for k right-hand sides (x: n*ldx, y: n*ldy elements)."""
+        self.eng._check(lib().spmv_axt_gather(
+            self.eng.h, self.plan.h, _dptr(vals if vals is not None else self.aval),
+            _dptr(self.acol), _dptr(self.sec), _dptr(x), ldx or k, _dptr(y), ldy or k, k),
+            "spmv_axt_gather")
+
+    def grad_vals(self, x, ybar, vbar) -> None:
+        """vbar[s] = ybar[row(s)] * x[acol[s]] over the slots."""
+        self.eng._check(lib().spmv_axt_grad_vals(self.eng.h, self.plan.h, _dptr(self.acol),
+                                                 _dptr(self.sec), _dptr(x), _dptr(ybar),
+                                                 _dptr(vbar)), "spmv_axt_grad_vals")
+
+
+def transposed_slot_map(m, a, at_csr, perm, at) -> np.ndarray:
+    """permT[sT] = slot in A's layout of the entry held by slot sT of A^T's
+    layout (PAD on padding)."""
+    pt = np.full(at.lenAX // 2, PAD, np.uint32)
+    pt[at.nzslot] = a.nzslot[perm]
+    return pt
+
+
+class AxtAdjoint:
+    """The transposed tile map of an AXT matrix, built once: the AXT layout of
+    A^T (same tile height) plus the slot permutation from A's slot order.
+    ``adjoint`` computes X_bar = A^T Y_bar with the column-map kernel; the
+    values of A^T follow A's values through ``spmv_permute_vals``."""
+
+    def __init__(self, eng: Engine, m, fwd: DeviceAXTGather):
+        self.eng, self.fwd = eng, fwd
+        at_csr, perm = m.transpose()
+        zero = np.zeros(m.nrows)
+        at = H_AXT(at_csr, zero, th=fwd.a.tileH, want_acol=True)
+        self.at = at
+        self.permT = eng.upload(transposed_slot_map(m, fwd.a, at_csr, perm, at))
+        self.nslots_t = at.lenAX // 2
+        self.gT = DeviceAXTGather(eng, at, fwd.val_type, fwd.x_type)
+        self.refresh(fwd.aval)
+
+    def refresh(self, vals) -> None:
+        """Moves A's slot values (device) into A^T's slot order."""
+        self.eng._check(lib().spmv_permute_vals(self.eng.h, self.nslots_t, self.fwd.val_type,
+                                                _dptr(self.permT), _dptr(vals),
+                                                _dptr(self.gT.aval)), "spmv_permute_vals")
+
+    def adjoint(self, ybar, xbar, k: int = 1) -> None:
+        self.gT.matvec(ybar, xbar, k)
+
+
+def H_AXT(m, x, **kw):
+    from . import host as H
+    return H.AXT(m, x, **kw)
+
+
+def refresh_x(dev_axt: "DeviceAXT", acol_dev, x) -> None:
+    """Rewrites the embedded x of a reference-layout AXT on the device."""
+    dev_axt.eng._check(lib().spmv_axt_refresh_x(dev_axt.eng.h, dev_axt.plan.h, _dptr(acol_dev),
+                                                _dptr(x), _dptr(dev_axt.ax)), "spmv_axt_refresh_x")
+
+
+def grad_vals_embedded(dev_axt: "DeviceAXT", ybar, vbar) -> None:
+    dev_axt.eng._check(lib().spmv_axt_grad_vals_embedded(dev_axt.eng.h, dev_axt.plan.h,
+                                                         _dptr(dev_axt.ax), _dptr(dev_axt.sec),
+                                                         _dptr(ybar), _dptr(vbar)),
+                       "spmv_axt_grad_vals_embedded")

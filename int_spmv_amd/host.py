@@ -57,7 +57,8 @@ class _AXT(C.Structure):
                 ("tileHW", C.c_uint32), ("tileH", C.c_uint32), ("logTH", C.c_uint32),
                 ("tileN", C.c_uint32), ("lenAX", C.c_uint32), ("lenSEC", C.c_uint32),
                 ("lenCON", C.c_uint32), ("log", C.c_uint32), ("ax", f64p), ("sec", u32p),
-                ("con", u32p), ("bs", C.c_uint32), ("colN", C.c_uint32), ("acol", u32p)]
+                ("con", u32p), ("bs", C.c_uint32), ("colN", C.c_uint32), ("acol", u32p),
+                ("nzslot", u32p)]
 
 
 _lib = None
@@ -76,6 +77,7 @@ def lib():
         L.intspmv_csr_from_coo.argtypes = [C.c_uint32, C.c_size_t, u32p, u32p, f64p,
                                            C.POINTER(_CSR)]
         L.intspmv_csr_stats.argtypes = [C.POINTER(_CSR), C.c_char_p]
+        L.intspmv_csr_transpose.argtypes = [C.POINTER(_CSR), C.POINTER(_CSR), u32p]
         L.intspmv_csr_format_data.argtypes = [C.POINTER(_CSR), C.POINTER(FormatData)]
         L.intspmv_csr_format_data.restype = None
         L.intspmv_axc_build.argtypes = [C.POINTER(_CSR), f64p, C.POINTER(_AXC),
@@ -248,6 +250,14 @@ class CSR:
         return dict(nrows=s.nrows, nnz=s.nnz, rmin=s.rmin, rave=s.rave, rmax=s.rmax,
                     rsd=s.rsd, bw=s.bw)
 
+    def transpose(self):
+        """(A^T, perm): perm[k'] = index in A of entry k' of A^T."""
+        s = _CSR()
+        perm = np.empty(max(self.nnz, 1), np.uint32)
+        _check(lib().intspmv_csr_transpose(C.byref(self._s), C.byref(s), _ptr(perm, C.c_uint32)),
+               "transpose")
+        return CSR(s), perm[: self.nnz]
+
     def format_data(self) -> FormatData:
         fd = FormatData()
         lib().intspmv_csr_format_data(C.byref(self._s), C.byref(fd))
@@ -363,6 +373,11 @@ class AXT:
     sec = property(lambda self: _view(self._s.sec, self._s.lenSEC, np.uint32))
     con = property(lambda self: _view(self._s.con, self._s.lenCON, np.uint32))
     acol = property(lambda self: _view(self._s.acol, self._s.lenAX // 2, np.uint32))
+    nzslot = property(lambda self: _view(self._s.nzslot, self._s.nnz, np.uint32))
+
+    def value_plane(self) -> np.ndarray:
+        """Stored values in slot order (slot = T*tileH*8 + R*8 + C)."""
+        return np.ascontiguousarray(self.ax.reshape(-1, 2, 8)[:, 0, :]).reshape(-1)
 
 
 def wtime() -> float:

@@ -1,0 +1,170 @@
+"""GPU: column-map AXT (multi-RHS), the transposed tile map (adjoint), the
+value gradient, device-side x refresh, and the autograd Function checked by
+central finite differences.  Oracle: CSR-naive (cf_CSR) and the sequential
+CSR transpose product."""
+import numpy as np
+import pytest
+
+import oracle as O
+from int_spmv_amd import device as D
+from int_spmv_amd import host as H
+
+from .util import bitwise_equal, csr_long_rows, csr_with_empty_rows, scaled
+
+pytestmark = pytest.mark.gpu
+
+
+def _host(t, n):
+    import torch
+    torch.cuda.synchronize()
+    return t.reshape(-1)[:n].double().cpu().numpy()
+
+
+def _nan(eng, n, dtype=np.float64):
+    t = eng.empty(n, dtype)
+    t.fill_(float("nan"))
+    return t
+
+
+def _mats():
+    return [("powerlaw", H.CSR.powerlaw(n=1 << 13, target_nnz=200_000, maxlen=4096, seed=2)),
+            ("empty_rows", csr_with_empty_rows()),
+            ("long_rows", csr_long_rows())]
+
+
+@pytest.fixture(scope="module")
+def mats():
+    return _mats()
+
+
+@pytest.mark.parametrize("th", [1, 4])
+def test_gather_k1_equals_embedded(engine, mats, th):
+    """Same tiles, same trees: for every row inside a normal block the
+    column-map product of one x has the bits of the embedded (reference
+    layout) product; long rows agree within the FP64 tolerance."""
+    for name, m in mats:
+        x = H.uniform(m.nrows, seed=3)
+        a = H.AXT(m, x, th=th, want_acol=True)
+        g = D.DeviceAXTGather(engine, a)
+        e = D.DeviceAXT(engine, a)
+        y1, y2 = _nan(engine, m.nrows), _nan(engine, m.nrows)
+        g.matvec(engine.upload(x), y1)
+        e.int_axt(y2)
+        g1, g2 = _host(y1, m.nrows), _host(y2, m.nrows)
+        per = 8 if th == 1 else th
+        units = (np.diff(m.row.astype(np.int64)) + per - 1) // per
+        ok = units <= (64 if th == 1 else 64)
+        assert bitwise_equal(g1[ok], g2[ok]), (name, th)
+        assert scaled(m, x, O.cf_csr(m.row, m.col, m.val, x), g1) <= 1e-12, (name, th)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 13, 32])
+@pytest.mark.parametrize("th", [1, 8])
+def test_gather_multi_rhs(engine, mats, k, th):
+    for name, m in mats:
+        n = m.nrows
+        X = np.stack([H.uniform(n, seed=100 + j) for j in range(k)], axis=1)  # n x k
+        a = H.AXT(m, np.zeros(n), th=th, want_acol=True)
+        g = D.DeviceAXTGather(engine, a)
+        Y = _nan(engine, n * k)
+        g.matvec(engine.upload(X.reshape(-1)), Y, k=k)
+        Yg = _host(Y, n * k).reshape(n, k)
+        for j in range(k):
+            yr = O.cf_csr(m.row, m.col, m.val, X[:, j])
+            assert scaled(m, X[:, j], yr, Yg[:, j]) <= 1e-12, (name, k, th, j)
+
+
+@pytest.mark.parametrize("vt,xt,tol", [(D.F32, D.F32, 1e-6), (D.BF16, D.F32, 1e-2)])
+def test_gather_precision(engine, mats, vt, xt, tol):
+    for name, m in mats:
+        n, k = m.nrows, 4
+        X = np.stack([H.uniform(n, seed=7 + j) for j in range(k)], axis=1)
+        a = H.AXT(m, np.zeros(n), th=1, want_acol=True)
+        g = D.DeviceAXTGather(engine, a, val_type=vt, x_type=xt)
+        Y = _nan(engine, n * k, np.float32)
+        g.matvec(engine.upload(X.astype(np.float32).reshape(-1)), Y, k=k)
+        Yg = _host(Y, n * k).reshape(n, k)
+        for j in range(k):
+            yr = O.cf_csr(m.row, m.col, m.val, X[:, j])
+            assert scaled(m, X[:, j], yr, Yg[:, j]) <= tol, (name, vt, j)
+
+
+@pytest.mark.parametrize("th", [1, 2, 8])
+def test_adjoint_and_grad_vals(engine, mats, th):
+    for name, m in mats:
+        n = m.nrows
+        x = H.uniform(n, seed=11)
+        ybar = H.normal(n, seed=12)
+        a = H.AXT(m, x, th=th, want_acol=True)
+        fwd = D.DeviceAXTGather(engine, a)
+        adj = D.AxtAdjoint(engine, m, fwd)
+        xb = _nan(engine, n)
+        adj.adjoint(engine.upload(ybar), xb)
+        got = _host(xb, n)
+        ref = O.csr_transpose_product(m.row, m.col, m.val, ybar)
+        at, _ = m.transpose()
+        assert scaled(at, ybar, ref, got) <= 1e-12, (name, th)
+        vb = _nan(engine, a.lenAX // 2)
+        fwd.grad_vals(engine.upload(x), engine.upload(ybar), vb)
+        g = _host(vb, a.lenAX // 2)
+        rows = np.repeat(np.arange(n), np.diff(m.row.astype(np.int64)))
+        want = np.zeros(a.lenAX // 2)
+        want[a.nzslot] = ybar[rows] * x[m.col]
+        assert bitwise_equal(g, want), (name, th)
+        # embedded layout: the tiles already carry x[col]
+        e = D.DeviceAXT(engine, a)
+        vb2 = _nan(engine, a.lenAX // 2)
+        D.grad_vals_embedded(e, engine.upload(ybar), vb2)
+        assert bitwise_equal(_host(vb2, a.lenAX // 2), want), (name, th)
+
+
+def test_refresh_x_on_device(engine, mats):
+    for name, m in mats:
+        x0, x1 = H.uniform(m.nrows, seed=1), H.uniform(m.nrows, seed=2)
+        a = H.AXT(m, x0, th=4, want_acol=True)
+        e = D.DeviceAXT(engine, a)
+        D.refresh_x(e, engine.upload(a.acol), engine.upload(x1))
+        y = _nan(engine, m.nrows)
+        e.int_axt(y)
+        b = H.AXT(m, x1, th=4)
+        import torch
+        torch.cuda.synchronize()
+        assert bitwise_equal(e.ax.cpu().numpy(), b.ax), name
+        assert scaled(m, x1, O.cf_csr(m.row, m.col, m.val, x1), _host(y, m.nrows)) <= 1e-12
+
+
+def test_autograd_finite_differences(engine):
+    """L(v, x) = ybar^T A(v) x is bilinear, so central differences along any
+    direction equal the directional derivative up to rounding."""
+    import torch
+
+    from int_spmv_amd import autograd as AG
+    m = H.CSR.powerlaw(n=1 << 12, target_nnz=80_000, maxlen=2000, seed=5)
+    n = m.nrows
+    op = AG.AxtOperator(engine, m, th=1)
+    dev = engine.torch_device
+    v = op.initial_values().requires_grad_(True)
+    x = torch.tensor(H.uniform(n, seed=21), device=dev, requires_grad=True)
+    ybar = torch.tensor(H.normal(n, seed=22), device=dev)
+    L = (AG.spmv(v, x, op) * ybar).sum()
+    L.backward()
+    gv, gx = v.grad.detach(), x.grad.detach()
+    # x_bar against the CSR transpose product
+    ref = O.csr_transpose_product(m.row, m.col, m.val, ybar.cpu().numpy())
+    at, _ = m.transpose()
+    assert scaled(at, ybar.cpu().numpy(), ref, gx.cpu().numpy()) <= 1e-12
+    occupied = op.fwd.acol.cpu().numpy().view(np.uint32) != D.PAD
+    rng = np.random.default_rng(0)
+    h = 1e-6
+    worst = 0.0
+    with torch.no_grad():
+        for _ in range(64):
+            dv = torch.tensor(rng.standard_normal(len(occupied)) * occupied, device=dev)
+            dx = torch.tensor(rng.standard_normal(n), device=dev)
+            lp = (AG.spmv(v + h * dv, x + h * dx, op) * ybar).sum()
+            lm = (AG.spmv(v - h * dv, x - h * dx, op) * ybar).sum()
+            fd = ((lp - lm) / (2 * h)).item()
+            an = ((gv * dv).sum() + (gx * dx).sum()).item()
+            scale = ((gv * dv).abs().sum() + (gx * dx).abs().sum()).item()
+            worst = max(worst, abs(fd - an) / scale)
+    assert worst < 1e-6, worst

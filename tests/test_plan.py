@@ -1,0 +1,60 @@
+"""Host-only checks of the work decomposition (intspmv_plan_layout): every
+row is covered exactly once, every unit exactly once, limits hold, split
+pieces are consecutive and numbered, long blocks precede normal ones."""
+import numpy as np
+import pytest
+
+from int_spmv_amd import device as D
+from int_spmv_amd import host as H
+
+from .util import csr_long_rows, csr_with_empty_rows
+
+
+def _check(uptr, umax, rmax, lu, lmax):
+    b, s, nlw = D.plan_layout(uptr, umax, rmax, lu, lmax)
+    n = len(uptr) - 1
+    real = b[b[:, 2] != 0xFFFFFFFF]
+    assert nlw % 8 == 0 and nlw * 4 <= len(b)
+    # long section first
+    assert np.all(b[: nlw * 4, 5] >= 1) and np.all(b[nlw * 4:, 5] == 0)
+    rows = np.zeros(n, np.int64)
+    units = np.zeros(int(uptr[-1]), np.int64)
+    for ub, ue, rb, re, b0, k, piece, _ in real:
+        if k <= 1:
+            rows[rb:re] += 1
+        if k == 0:
+            assert ue - ub <= umax and re - rb <= rmax
+            assert ub == uptr[rb] and ue == uptr[re]
+            assert np.all(np.diff(uptr[rb:re + 1].astype(np.int64)) <= lu)
+        else:
+            assert re == rb + 1 and ue - ub <= lmax
+        units[ub:ue] += 1
+    for row, p0, k, _ in s:
+        pieces = real[(real[:, 5] == k) & (real[:, 2] == row)]
+        assert len(pieces) == k
+        assert list(pieces[:, 6]) == list(range(p0, p0 + k))
+        assert pieces[0, 0] == uptr[row] and pieces[-1, 1] == uptr[row + 1]
+        assert np.all(pieces[1:, 0] == pieces[:-1, 1])
+        rows[row] += 1
+    assert np.all(rows == 1)
+    assert np.all(units == 1)
+    return b, s
+
+
+@pytest.mark.parametrize("m", [csr_with_empty_rows(), csr_long_rows(),
+                               H.CSR.powerlaw(n=20000, target_nnz=600000, maxlen=8192, seed=1),
+                               H.CSR.laplace2d(64)])
+@pytest.mark.parametrize("params", [(128, 256, 64, 1024), (256, 256, 64, 1024),
+                                    (512, 256, 256, 8192), (64, 16, 8, 40)])
+def test_plan_layout_invariants(m, params):
+    _check(m.row, *params)
+
+
+def test_plan_layout_splits_and_order():
+    m = csr_long_rows()
+    b, s = _check(m.row, 512, 256, 256, 8192)
+    # rows of 20000, 9000, 12000 and 30000 entries exceed 8192: 3, 2, 2, 4 pieces,
+    # ordered by row, numbered consecutively
+    assert s[:, 0].tolist() == [10, 11, 5000, 29999]
+    assert s[:, 2].tolist() == [3, 2, 2, 4]
+    assert s[:, 1].tolist() == [0, 3, 5, 7]
