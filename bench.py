@@ -152,6 +152,31 @@ def main():
                                    "vs_fp64": round(head_ev / e, 2)})
             formats[nm]["_y"] = y
             del d
+        # column-map form: 1 and 8 right-hand sides, adjoint on the transposed map
+        ag = H.AXT(m, x, th=1, want_acol=True)
+        g = D.DeviceAXTGather(eng, ag)
+        y = eng.empty(n)
+        w, e = timed(lambda: g.matvec(xd, y), 20, 3, torch)
+        fwd_g = e
+        record("AXT_GATHER_H1_F64_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC), w, e)
+        k8 = 8
+        X8 = torch.rand(n * k8, dtype=torch.float64, device=eng.torch_device)
+        Y8 = eng.empty(n * k8)
+        w, e = timed(lambda: g.matvec(X8, Y8, k=k8), 10, 2, torch)
+        record("AXT_GATHER_H1_F64_K8", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, k=k8), w, e,
+               {"GFLOPs": round(M.flops(nnz, k8) / e / 1e9, 1)})
+        adj = D.AxtAdjoint(eng, m, g)
+        yb = torch.randn(n, dtype=torch.float64, device=eng.torch_device)
+        xb = eng.empty(n)
+        w, e = timed(lambda: adj.adjoint(yb, xb), 20, 3, torch)
+        at = adj.at
+        record("AXT_ADJOINT_H1_F64", M.bytes_axt_gather(n, at.lenAX, at.lenSEC), w, e,
+               {"vs_forward_gather": round(e / fwd_g, 3), "vs_forward_embedded": round(e / head_ev, 3)})
+        vb = eng.empty(ag.lenAX // 2)
+        w, e = timed(lambda: g.grad_vals(xd, yb, vb), 20, 3, torch)
+        record("AXT_GRAD_VALS_H1_F64", (ag.lenAX // 2) * (4 + 8) + ag.lenSEC * 4 + 2 * n * 8, w, e,
+               {"vs_forward_gather": round(e / fwd_g, 3)})
+        del g, adj, ag, X8, Y8, vb
         dc = D.DeviceCSR(eng, m)
         y = eng.empty(n)
         w, e = timed(lambda: dc.spmv(xd, y), 20, 3, torch)

@@ -94,75 +94,88 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
     AT lacc[KC];
 #pragma unroll
     for (int j = 0; j < KC; j++) lacc[j] = 0;
-    // tiles of this block
+    // tiles of this block; UG steps of 16 tiles are loaded before their
+    // gathers are issued, so a lane keeps 2*UG gathers in flight
+    constexpr int UG = KC >= 8 ? 1 : (KC >= 4 ? 2 : 4);
     const uint32_t T0 = H1 ? d.ubeg : d.ubeg / 8, T1 = H1 ? d.uend : (d.uend + 7) / 8;
-    for (uint32_t Tb = T0; Tb < T1; Tb += 16) {
-        const uint32_t Tr = Tb + lane / 4;
-        const size_t T = Tr < T1 ? Tr : T1 - 1;
-        AT a0[KC], a1[KC];
+    for (uint32_t Tbb = T0; Tbb < T1; Tbb += 16 * UG) {
+        AT a0[UG][KC], a1[UG][KC];
 #pragma unroll
-        for (int j = 0; j < KC; j++) a0[j] = a1[j] = 0;
+        for (int u = 0; u < UG; u++)
+#pragma unroll
+            for (int j = 0; j < KC; j++) a0[u][j] = a1[u][j] = 0;
         for (uint32_t R = 0; R < th; R++) {
-            GLoad<VT> v0, v1;
-            const size_t sidx = (T * th + R) * 8 + C;
-            GPair<VT>::ld(aval + sidx, v0, v1);
-            const u2v cc = ntload(reinterpret_cast<const u2v *>(acol + sidx));
-            AT x0[KC], x1[KC];
-            gather_x<XT, AT, KC>(x, cc[0], ldx, j0, kv, x0);
-            gather_x<XT, AT, KC>(x, cc[1], ldx, j0, kv, x1);
-            if (H1) {
+            GLoad<VT> v0[UG], v1[UG];
+            u2v cc[UG];
+#pragma unroll
+            for (int u = 0; u < UG; u++) {
+                const uint32_t Tr = Tbb + 16 * u + lane / 4;
+                const size_t T = Tr < T1 ? Tr : T1 - 1;
+                const size_t sidx = (T * th + R) * 8 + C;
+                GPair<VT>::ld(aval + sidx, v0[u], v1[u]);
+                cc[u] = ntload(reinterpret_cast<const u2v *>(acol + sidx));
+            }
+#pragma unroll
+            for (int u = 0; u < UG; u++) {
+                AT x0[KC], x1[KC];
+                gather_x<XT, AT, KC>(x, cc[u][0], ldx, j0, kv, x0);
+                gather_x<XT, AT, KC>(x, cc[u][1], ldx, j0, kv, x1);
 #pragma unroll
                 for (int j = 0; j < KC; j++) {
-                    a0[j] = (AT)v0 * x0[j];
-                    a1[j] = (AT)v1 * x1[j];
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < KC; j++) {
-                    a0[j] = gfma<AT>((AT)v0, x0[j], a0[j]);
-                    a1[j] = gfma<AT>((AT)v1, x1[j], a1[j]);
+                    if (H1) {
+                        a0[u][j] = (AT)v0[u] * x0[j];
+                        a1[u][j] = (AT)v1[u] * x1[j];
+                    } else {
+                        a0[u][j] = gfma<AT>((AT)v0[u], x0[j], a0[u][j]);
+                        a1[u][j] = gfma<AT>((AT)v1[u], x1[j], a1[u][j]);
+                    }
                 }
             }
         }
-        if (H1) {
-            const bool ok = Tr < T1;
-            if (d.k != 0) {
-                if (ok)
 #pragma unroll
-                    for (int j = 0; j < KC; j++) lacc[j] = lacc[j] + (a0[j] + a1[j]);
-                continue;
-            }
-            const uint32_t key = ntload(sec + T);
+        for (int u = 0; u < UG; u++) {
+            const uint32_t Tr = Tbb + 16 * u + lane / 4;
+            const size_t T = Tr < T1 ? Tr : T1 - 1;
+            if (H1) {
+                const bool ok = Tr < T1;
+                if (d.k != 0) {
+                    if (ok)
 #pragma unroll
-            for (int j = 0; j < KC; j++) {
-                AT p0 = a0[j], p1 = a1[j];
-                p0 = p0 + dpp_any<AT, DPP_QUAD_XOR2>(p0);
-                p1 = p1 + dpp_any<AT, DPP_QUAD_XOR2>(p1);
-                p0 = p0 + dpp_any<AT, DPP_QUAD_XOR1>(p0);
-                p1 = p1 + dpp_any<AT, DPP_QUAD_XOR1>(p1);
-                if (ok && q == 0) s_val[w][(Tr - d.ubeg) * KC + j] = p0 + p1;
-            }
-            if (ok && q == 0) s_key[w][Tr - d.ubeg] = key;
-        } else {
-            const uint32_t u0 = 8 * Tr + C;
-            const bool ok0 = Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
-            const bool ok1 = Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
-            if (d.k != 0) {
+                        for (int j = 0; j < KC; j++) lacc[j] = lacc[j] + (a0[u][j] + a1[u][j]);
+                    continue;
+                }
+                const uint32_t key = ntload(sec + T);
 #pragma unroll
                 for (int j = 0; j < KC; j++) {
-                    if (ok0) lacc[j] = lacc[j] + a0[j];
-                    if (ok1) lacc[j] = lacc[j] + a1[j];
+                    AT p0 = a0[u][j], p1 = a1[u][j];
+                    p0 = p0 + dpp_any<AT, DPP_QUAD_XOR2>(p0);
+                    p1 = p1 + dpp_any<AT, DPP_QUAD_XOR2>(p1);
+                    p0 = p0 + dpp_any<AT, DPP_QUAD_XOR1>(p0);
+                    p1 = p1 + dpp_any<AT, DPP_QUAD_XOR1>(p1);
+                    if (ok && q == 0) s_val[w][(Tr - d.ubeg) * KC + j] = p0 + p1;
                 }
-                continue;
-            }
-            const u2v kk = ntload(reinterpret_cast<const u2v *>(sec + 8 * T + C));
+                if (ok && q == 0) s_key[w][Tr - d.ubeg] = key;
+            } else {
+                const uint32_t u0 = 8 * Tr + C;
+                const bool ok0 = Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
+                const bool ok1 = Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
+                if (d.k != 0) {
 #pragma unroll
-            for (int j = 0; j < KC; j++) {
-                if (ok0) s_val[w][(u0 - d.ubeg) * KC + j] = a0[j];
-                if (ok1) s_val[w][(u0 + 1 - d.ubeg) * KC + j] = a1[j];
+                    for (int j = 0; j < KC; j++) {
+                        if (ok0) lacc[j] = lacc[j] + a0[u][j];
+                        if (ok1) lacc[j] = lacc[j] + a1[u][j];
+                    }
+                    continue;
+                }
+                const u2v kk = ntload(reinterpret_cast<const u2v *>(sec + 8 * T + C));
+#pragma unroll
+                for (int j = 0; j < KC; j++) {
+                    if (ok0) s_val[w][(u0 - d.ubeg) * KC + j] = a0[u][j];
+                    if (ok1) s_val[w][(u0 + 1 - d.ubeg) * KC + j] = a1[u][j];
+                }
+                if (ok0) s_key[w][u0 - d.ubeg] = kk[0];
+                if (ok1) s_key[w][u0 + 1 - d.ubeg] = kk[1];
             }
-            if (ok0) s_key[w][u0 - d.ubeg] = kk[0];
-            if (ok1) s_key[w][u0 + 1 - d.ubeg] = kk[1];
         }
     }
     if (d.k != 0) {
