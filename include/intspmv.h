@@ -140,7 +140,9 @@ int spmv_axt(intspmv_ctx *ctx, const intspmv_plan *plan, const void *ax, const v
  * reference embeds x (intSpmv_main.c:1171, 1207), which fixes one x per build. */
 int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval,
                     const uint32_t *acol, const uint32_t *sec, const void *x, uint32_t ldx,
-                    void *y, uint32_t ldy, uint32_t k);
+                    void *y, uint32_t ldy, uint32_t k, uint32_t kc);
+/* kc: right-hand sides per pass over the matrix (1, 2, 4 or 8); 0 = the largest
+ * power of two <= min(k, 8) for each pass. */
 /* Value gradient on the pattern: vbar[s] = ybar[row(s)] * x[acol[s]] (0 on
  * padding), one entry per slot (lenAX/2).  F64 or F32. */
 int spmv_axt_grad_vals(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_t *acol,
@@ -158,6 +160,36 @@ int spmv_permute_vals(intspmv_ctx *ctx, uint32_t n, int val_type, const uint32_t
  * vec[col] step, intSpmv_main.c:1171, 1207), so x can change without a rebuild. */
 int spmv_axt_refresh_x(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_t *acol,
                        const double *x, double *ax);
+
+/* ---- batches ------------------------------------------------------------
+ * Many AXT NOC matrices in column-map form, each with its own tile height,
+ * 1..32 right-hand sides and right-hand-side blocking kc (1, 2, 4, 8; 0 = the
+ * largest power of two <= min(k, 8)), run by ONE persistent kernel (plus one
+ * fix-up kernel when some row is longer than a wave streams alone).  The work
+ * list is built once on the host.  intspmv_batch_graph_capture records the
+ * launch in a hipGraph once; intspmv_batch_graph_launch replays it.  Pointers
+ * in intspmv_batch_matrix are device pointers except con_host.  All matrices of
+ * a batch share val_type / x_type (F64/F64, F32/F32 or BF16/F32).  No
+ * reference counterpart (the reference times one matrix and one x). */
+typedef struct {
+    const void *aval;          /* slot-order values */
+    const uint32_t *acol, *sec;
+    const void *x;             /* n x k, leading dimension ldx */
+    void *y;                   /* n x k, leading dimension ldy */
+    uint32_t ldx, ldy, k, kc;
+    uint32_t nrows, tileH, tileN, lenAX, colN;
+    uint32_t umax;             /* units per work block (8..128; 0 = 128) */
+    const uint32_t *con_host;  /* str_matAXT.con of this matrix */
+} intspmv_batch_matrix;
+typedef struct intspmv_batch intspmv_batch;
+int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, uint32_t nmat,
+                         const intspmv_batch_matrix *mats, intspmv_batch **batch);
+int intspmv_batch_launch(intspmv_ctx *ctx, const intspmv_batch *batch);
+int intspmv_batch_graph_capture(intspmv_ctx *ctx, intspmv_batch *batch);
+int intspmv_batch_graph_launch(intspmv_ctx *ctx, const intspmv_batch *batch);
+int intspmv_batch_info(const intspmv_batch *batch, uint32_t *nitems, uint32_t *grid,
+                       uint32_t *nsplit);
+int intspmv_batch_destroy(intspmv_batch *batch);
 
 int intspmv_plan_destroy(intspmv_plan *plan);
 /* Host-only inspection of the work decomposition a plan uses for a row ->

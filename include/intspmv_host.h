@@ -142,6 +142,21 @@ int intspmv_axt_lengths(const intspmv_csr *m, uint32_t thw, uint32_t th, const c
                         uint32_t *con, uint32_t *tileN, uint32_t *lenAX, uint32_t *lenSEC,
                         uint32_t *colN);
 
+/* ---- autotuner model ------------------------------------------------------
+ * Ranks AXT NOC tile heights th in {1, 2, 4, 8, 16} and right-hand-side
+ * blockings kc in {1, 2, 4, 8} (kc <= k) for a column-map product with k
+ * right-hand sides, sv-byte values and sx-byte x, by modelled bytes moved:
+ *   passes * (slots * (sv + 4 + kc * sx) + 4 * lenSEC) + k * n * sx,
+ * passes = ceil(k / kc), slots = lenAX / 2 from the exact length formula of
+ * getArraysLenAXT (no arrays are built).  Fills up to cap entries of out in
+ * ascending cost; *n receives the number of candidates. */
+typedef struct {
+    uint32_t tileH, kc;
+    uint64_t slots, bytes;
+} intspmv_tune_choice;
+int intspmv_tune_axt(const intspmv_csr *m, uint32_t k, uint32_t sv, uint32_t sx,
+                     intspmv_tune_choice *out, uint32_t cap, uint32_t *n);
+
 /* ---- synthetic inputs --------------------------------------------------- */
 /* Deterministic generator: splitmix64-seeded xoshiro256**. */
 void intspmv_fill_uniform(double *v, size_t n, double lo, double hi, uint64_t seed);

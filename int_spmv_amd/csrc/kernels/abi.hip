@@ -225,11 +225,11 @@ int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size
 // ceil(len / long_max) near-equal pieces combined by the fix-up kernel.  The
 // long blocks come first, longest first, padded to whole XCD rounds of
 // workgroups; the normal blocks follow in row order.  Pure host code.
-static int build_blocks(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
-                        uint32_t long_units, uint32_t long_max,
-                        std::vector<ispv::BlockDesc> &blocks,
-                        std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg,
-                        uint32_t &nlong, uint32_t &npieces)
+int ispv_build_blocks(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
+                      uint32_t long_units, uint32_t long_max,
+                      std::vector<ispv::BlockDesc> &blocks,
+                      std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg, uint32_t &nlong,
+                      uint32_t &npieces)
 {
     if (!uptr || umax == 0 || rmax == 0 || long_units >= umax || long_max <= long_units)
         return INTSPMV_ERR_ARG;
@@ -306,7 +306,7 @@ extern "C" int intspmv_plan_layout(uint32_t nrows, const uint32_t *uptr, uint32_
     std::vector<ispv::BlockDesc> b;
     std::vector<ispv::SplitDesc> sp;
     uint32_t nlw = 0, nlong = 0, npieces = 0;
-    int st = build_blocks(nrows, uptr, umax, rmax, long_units, long_max, b, sp, nlw, nlong,
+    int st = ispv_build_blocks(nrows, uptr, umax, rmax, long_units, long_max, b, sp, nlw, nlong,
                           npieces);
     if (st) return st;
     if (nblocks) *nblocks = (uint32_t)b.size();
@@ -328,7 +328,7 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
     std::vector<ispv::BlockDesc> blocks;
     std::vector<ispv::SplitDesc> splits;
     uint32_t nlw = 0, nlong = 0, npieces = 0;
-    int st = build_blocks(nrows, uptr, umax, rmax, long_units, long_max, blocks, splits, nlw,
+    int st = ispv_build_blocks(nrows, uptr, umax, rmax, long_units, long_max, blocks, splits, nlw,
                           nlong, npieces);
     if (st) return st;
     intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));

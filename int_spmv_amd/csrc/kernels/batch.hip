@@ -1,0 +1,326 @@
+// batch.hip -- many AXT matrices, 1..32 right-hand sides each, in one
+// persistent launch that can be captured once in a hipGraph and replayed.
+//
+// Each matrix is in column-map form (gather.hip) with its own tile height,
+// its own plan blocks and its own right-hand-side blocking kc (chosen per
+// matrix by the autotuner).  The host flattens everything into one list of
+// work items (matrix, block, first right-hand side), longest first; a grid
+// sized to the resident capacity of the device walks the list with a static
+// stride (wave w takes items w, w + W, ...), so the schedule is fixed and the
+// results are deterministic.  Rows split across items meet in a batch-wide
+// partial-sum scratch that one fix-up kernel reduces afterwards.  The reference
+// has no counterpart (it times one matrix, one x).
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "gather_block.hpp"
+#include "reduce.hpp"
+
+namespace ispv {
+
+struct BatchMat {
+    const void *aval;
+    const uint32_t *acol, *sec;
+    const void *x;
+    void *y;
+    uint32_t nrows, th, k, ldx, ldy, kc, piece_off, pad;
+};
+struct BatchItem {
+    uint32_t mat, block, j0, pad;
+};
+struct BatchSplit {
+    uint32_t mat, row, piece, k;
+};
+
+} // namespace ispv
+
+struct intspmv_batch {
+    intspmv_ctx *ctx;
+    int val_type, x_type;
+    uint32_t nmat, nitems, nblocks, nsplit, grid;
+    ispv::BatchMat *d_mats;
+    ispv::BatchItem *d_items;
+    ispv::BlockDesc *d_blocks;
+    ispv::BatchSplit *d_splits;
+    double *d_partial;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    hipStream_t cap_stream;
+};
+
+
+namespace ispv {
+
+template <typename VT, typename XT, typename AT>
+__global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
+    const BatchItem *__restrict__ items, uint32_t nitems, const BlockDesc *__restrict__ blocks,
+    const BatchMat *__restrict__ mats, double *partial)
+{
+    __shared__ GatherLds<AT, 8> lds[WAVES];
+    const uint32_t w = threadIdx.x / WAVE;
+    const uint32_t total = gridDim.x * WAVES;
+    for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + w); it < nitems;
+         it += total) {
+        const BatchItem item = items[it];
+        const BatchMat md = mats[item.mat];
+        const BlockDesc d = blocks[item.block];
+        double *part = partial + (size_t)md.piece_off * PARTIAL_STRIDE;
+        const uint32_t kv = md.k - item.j0 < md.kc ? md.k - item.j0 : md.kc;
+        const VT *av = (const VT *)md.aval;
+        const XT *xx = (const XT *)md.x;
+        AT *yy = (AT *)md.y;
+        GatherLds<AT, 8> &L = lds[w];
+#define ISPV_B(KC)                                                                              \
+    if (md.kc == KC) {                                                                          \
+        if (md.th == 1)                                                                         \
+            gather_block<VT, XT, AT, KC, true>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy,  \
+                                               item.j0, kv, 1, md.nrows, part, L.val, L.key, L.seg); \
+        else                                                                                    \
+            gather_block<VT, XT, AT, KC, false>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy, \
+                                                item.j0, kv, md.th, md.nrows, part, L.val, L.key,  \
+                                                L.seg);                                         \
+    }
+        ISPV_B(1)
+        ISPV_B(2)
+        ISPV_B(4)
+        ISPV_B(8)
+#undef ISPV_B
+        wave_lds_sync(); // the next item reuses this wave's LDS
+    }
+}
+
+template <typename YT>
+__global__ void batch_fixup_kernel(const BatchSplit *__restrict__ splits, uint32_t nsplit,
+                                   const BatchMat *__restrict__ mats,
+                                   const double *__restrict__ partial)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, i = t / PARTIAL_STRIDE,
+                   j = t % PARTIAL_STRIDE;
+    if (i >= nsplit) return;
+    const BatchSplit sp = splits[i];
+    const BatchMat md = mats[sp.mat];
+    if (j >= md.k) return;
+    double acc = 0.0;
+    for (uint32_t q = 0; q < sp.k; q++)
+        acc = acc + partial[(size_t)(md.piece_off + sp.piece + q) * PARTIAL_STRIDE + j];
+    if (sp.row < md.nrows) ((YT *)md.y)[(size_t)sp.row * md.ldy + j] = (YT)acc;
+}
+
+} // namespace ispv
+
+using namespace ispv;
+
+
+extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, uint32_t nmat,
+                                    const intspmv_batch_matrix *mats, intspmv_batch **out)
+{
+    if (!ctx || !out || (nmat && !mats)) return INTSPMV_ERR_ARG;
+    *out = nullptr;
+    const bool f64 = val_type == INTSPMV_F64 && x_type == INTSPMV_F64;
+    const bool f32 = val_type == INTSPMV_F32 && x_type == INTSPMV_F32;
+    const bool b16 = val_type == INTSPMV_BF16 && x_type == INTSPMV_F32;
+    if (!f64 && !f32 && !b16) return INTSPMV_ERR_UNSUPPORTED;
+    std::vector<BatchMat> hm(nmat);
+    std::vector<BlockDesc> blocks;
+    std::vector<BatchSplit> splits;
+    struct Cand {
+        BatchItem item;
+        uint64_t cost;
+    };
+    std::vector<Cand> cand;
+    uint32_t piece_base = 0;
+    for (uint32_t i = 0; i < nmat; i++) {
+        const intspmv_batch_matrix &M = mats[i];
+        const uint32_t th = M.tileH;
+        if (th == 0 || M.k == 0 || M.k > PARTIAL_STRIDE || M.ldx < M.k || M.ldy < M.k ||
+            (M.nrows && (!M.con_host || !M.x || !M.y)))
+            return INTSPMV_ERR_ARG;
+        uint32_t kc = M.kc ? M.kc : (M.k >= 8 ? 8 : (M.k >= 4 ? 4 : (M.k >= 2 ? 2 : 1)));
+        if (kc != 1 && kc != 2 && kc != 4 && kc != 8) return INTSPMV_ERR_ARG;
+        const uint32_t umax = M.umax ? M.umax : ISPV_AXT_UMAX;
+        if (umax > ISPV_AXT_UMAX || umax < 8) return INTSPMV_ERR_ARG;
+        const uint32_t ths = 8 * th;
+        if ((uint64_t)M.tileN * 2 * ths != M.lenAX) return INTSPMV_ERR_ARG;
+        const uint32_t nunits = th == 1 ? M.tileN : M.colN;
+        std::vector<uint32_t> uptr((size_t)M.nrows + 1);
+        for (uint32_t r = 0; r < M.nrows; r++) {
+            uint32_t c = M.con_host[r], T = c / (2 * ths), wpos = c % (2 * ths);
+            uptr[r] = th == 1 ? T : T * 8 + wpos % 16;
+            if (wpos % 16 >= 8 || uptr[r] > nunits || (r && uptr[r] < uptr[r - 1]))
+                return INTSPMV_ERR_ARG;
+        }
+        uptr[M.nrows] = nunits;
+        uint32_t long_units = th == 1 ? umax / 2 : (512 / th < 4 ? 4 : (512 / th > umax / 2 ? umax / 2 : 512 / th));
+        std::vector<BlockDesc> mb;
+        std::vector<SplitDesc> ms;
+        uint32_t nlw = 0, nlong = 0, npieces = 0;
+        int st = ispv_build_blocks(M.nrows, uptr.data(), umax, 256, long_units,
+                                   th == 1 ? 1024 : 8192 / th, mb, ms, nlw, nlong, npieces);
+        if (st) return st;
+        hm[i] = {M.aval, M.acol, M.sec, M.x, M.y, M.nrows, th, M.k, M.ldx, M.ldy, kc, piece_base, 0};
+        for (const BlockDesc &d : mb) {
+            const uint32_t bi = (uint32_t)blocks.size();
+            blocks.push_back(d);
+            if (d.rbeg == 0xFFFFFFFFu) continue; // XCD padding of the per-matrix plan
+            const uint64_t slots = (uint64_t)(d.uend - d.ubeg) * (th == 1 ? 8 : th);
+            for (uint32_t j0 = 0; j0 < M.k; j0 += kc)
+                cand.push_back({{i, bi, j0, 0}, slots * (kc < M.k - j0 ? kc : M.k - j0) + 64});
+        }
+        for (const SplitDesc &sp : ms) splits.push_back({i, sp.row, sp.b0, sp.k});
+        piece_base += npieces;
+    }
+    // longest items first; static striding then gives every wave a similar mix
+    std::stable_sort(cand.begin(), cand.end(),
+                     [](const Cand &a, const Cand &b) { return a.cost > b.cost; });
+    std::vector<BatchItem> items(cand.size());
+    for (size_t i = 0; i < cand.size(); i++) items[i] = cand[i].item;
+
+    intspmv_batch *b = (intspmv_batch *)calloc(1, sizeof(intspmv_batch));
+    if (!b) return INTSPMV_ERR_NOMEM;
+    b->ctx = ctx;
+    b->val_type = val_type;
+    b->x_type = x_type;
+    b->nmat = nmat;
+    b->nitems = (uint32_t)items.size();
+    b->nblocks = (uint32_t)blocks.size();
+    b->nsplit = (uint32_t)splits.size();
+    int st = ispv_enter(ctx);
+    if (st) {
+        free(b);
+        return st;
+    }
+    int dev = ctx->device, cus = 0, per_cu = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) {
+        if (f64)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<double, double, double>, BLOCK_THREADS, 0);
+        else if (f32)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<float, float, float>, BLOCK_THREADS, 0);
+        else
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<uint16_t, float, float>, BLOCK_THREADS, 0);
+    }
+    if (e != hipSuccess || cus <= 0) {
+        ctx->last_err = e;
+        free(b);
+        return INTSPMV_ERR_HIP;
+    }
+    if (per_cu < 1) per_cu = 1;
+    uint32_t want = (b->nitems + WAVES - 1) / WAVES;
+    b->grid = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)(cus * per_cu), want));
+    auto up = [&](void **dst, const void *src, size_t bytes) {
+        if (e != hipSuccess) return;
+        e = hipMalloc(dst, bytes ? bytes : 16);
+        if (e == hipSuccess && bytes) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    up((void **)&b->d_mats, hm.data(), hm.size() * sizeof(BatchMat));
+    up((void **)&b->d_items, items.data(), items.size() * sizeof(BatchItem));
+    up((void **)&b->d_blocks, blocks.data(), blocks.size() * sizeof(BlockDesc));
+    up((void **)&b->d_splits, splits.data(), splits.size() * sizeof(BatchSplit));
+    if (e == hipSuccess)
+        e = hipMalloc(&b->d_partial, (size_t)(piece_base ? piece_base : 1) * PARTIAL_STRIDE * sizeof(double));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        intspmv_batch_destroy(b);
+        return INTSPMV_ERR_NOMEM;
+    }
+    *out = b;
+    return INTSPMV_OK;
+}
+
+static int batch_enqueue(const intspmv_batch *b, hipStream_t s)
+{
+    if (b->nitems) {
+        if (b->val_type == INTSPMV_F64)
+            batch_kernel<double, double, double><<<b->grid, BLOCK_THREADS, 0, s>>>(
+                b->d_items, b->nitems, b->d_blocks, b->d_mats, b->d_partial);
+        else if (b->val_type == INTSPMV_F32)
+            batch_kernel<float, float, float><<<b->grid, BLOCK_THREADS, 0, s>>>(
+                b->d_items, b->nitems, b->d_blocks, b->d_mats, b->d_partial);
+        else
+            batch_kernel<uint16_t, float, float><<<b->grid, BLOCK_THREADS, 0, s>>>(
+                b->d_items, b->nitems, b->d_blocks, b->d_mats, b->d_partial);
+    }
+    if (b->nsplit) {
+        dim3 g((b->nsplit * PARTIAL_STRIDE + 255) / 256);
+        if (b->val_type == INTSPMV_F64)
+            batch_fixup_kernel<double><<<g, 256, 0, s>>>(b->d_splits, b->nsplit, b->d_mats, b->d_partial);
+        else
+            batch_fixup_kernel<float><<<g, 256, 0, s>>>(b->d_splits, b->nsplit, b->d_mats, b->d_partial);
+    }
+    return hipGetLastError() == hipSuccess ? INTSPMV_OK : INTSPMV_ERR_HIP;
+}
+
+extern "C" int intspmv_batch_launch(intspmv_ctx *ctx, const intspmv_batch *b)
+{
+    if (!ctx || !b || b->ctx != ctx) return INTSPMV_ERR_ARG;
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    st = batch_enqueue(b, ctx->stream);
+    if (st) ctx->last_err = hipGetLastError();
+    return st;
+}
+
+extern "C" int intspmv_batch_graph_capture(intspmv_ctx *ctx, intspmv_batch *b)
+{
+    if (!ctx || !b || b->ctx != ctx) return INTSPMV_ERR_ARG;
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    if (b->exec) return INTSPMV_OK;
+    hipError_t e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        int est = batch_enqueue(b, b->cap_stream);
+        hipError_t e2 = hipStreamEndCapture(b->cap_stream, &b->graph);
+        e = est ? hipErrorUnknown : e2;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        return INTSPMV_ERR_HIP;
+    }
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_batch_graph_launch(intspmv_ctx *ctx, const intspmv_batch *b)
+{
+    if (!ctx || !b || b->ctx != ctx || !b->exec) return INTSPMV_ERR_ARG;
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    hipError_t e = hipGraphLaunch(b->exec, ctx->stream);
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        return INTSPMV_ERR_HIP;
+    }
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_batch_info(const intspmv_batch *b, uint32_t *nitems, uint32_t *grid,
+                                  uint32_t *nsplit)
+{
+    if (!b) return INTSPMV_ERR_ARG;
+    if (nitems) *nitems = b->nitems;
+    if (grid) *grid = b->grid;
+    if (nsplit) *nsplit = b->nsplit;
+    return INTSPMV_OK;
+}
+
+extern "C" int intspmv_batch_destroy(intspmv_batch *b)
+{
+    if (!b) return INTSPMV_OK;
+    if (b->ctx) ispv_enter(b->ctx);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    (void)hipFree(b->d_mats);
+    (void)hipFree(b->d_items);
+    (void)hipFree(b->d_blocks);
+    (void)hipFree(b->d_splits);
+    (void)hipFree(b->d_partial);
+    free(b);
+    return INTSPMV_OK;
+}

@@ -180,6 +180,12 @@ int ispv_split_fixup(const intspmv_plan *p, void *y, int y_is_f32, uint32_t k = 
 int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
                     uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_max,
                     intspmv_plan **out);
+#include <vector>
+// host-only block builder (abi.hip), shared by the single-matrix and batch plans
+int ispv_build_blocks(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
+                      uint32_t long_units, uint32_t long_max, std::vector<ispv::BlockDesc> &blocks,
+                      std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg, uint32_t &nlong,
+                      uint32_t &npieces);
 // grid size of a block plan
 static inline unsigned ispv_grid(const intspmv_plan *p)
 {

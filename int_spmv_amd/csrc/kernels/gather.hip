@@ -15,66 +15,11 @@
 //   spmv_axt_refresh_x   rewriting the embedded x of the reference layout on
 //                        the device instead of rebuilding on the host.
 #include "common.hpp"
+#include "gather_block.hpp"
 #include "reduce.hpp"
 
 namespace ispv {
 
-constexpr uint32_t G_UMAX = ISPV_AXT_UMAX; // units per normal block (shared AXT plan)
-constexpr uint32_t PAD = 0xFFFFFFFFu;
-
-template <typename T>
-struct GPair;
-template <>
-struct GPair<double> {
-    __device__ __forceinline__ static void ld(const double *p, double &a, double &b)
-    {
-        d2v v = ntload(reinterpret_cast<const d2v *>(p));
-        a = v[0];
-        b = v[1];
-    }
-};
-template <>
-struct GPair<float> {
-    __device__ __forceinline__ static void ld(const float *p, float &a, float &b)
-    {
-        f2v v = ntload(reinterpret_cast<const f2v *>(p));
-        a = v[0];
-        b = v[1];
-    }
-};
-template <>
-struct GPair<uint16_t> {
-    __device__ __forceinline__ static void ld(const uint16_t *p, float &a, float &b)
-    {
-        uint32_t v = ntload(reinterpret_cast<const uint32_t *>(p));
-        a = __uint_as_float(v << 16);
-        b = __uint_as_float(v & 0xFFFF0000u);
-    }
-};
-template <typename VT>
-using GLoad = typename std::conditional<std::is_same<VT, uint16_t>::value, float, VT>::type;
-
-template <typename AT>
-__device__ __forceinline__ AT gfma(AT a, AT b, AT c);
-template <>
-__device__ __forceinline__ double gfma<double>(double a, double b, double c) { return fma(a, b, c); }
-template <>
-__device__ __forceinline__ float gfma<float>(float a, float b, float c) { return fmaf(a, b, c); }
-
-// x values of one slot for KC right-hand sides starting at j0 (0 for padding)
-template <typename XT, typename AT, int KC>
-__device__ __forceinline__ void gather_x(const XT *x, uint32_t c, uint32_t ldx, uint32_t j0,
-                                         uint32_t kv, AT (&xv)[KC])
-{
-#pragma unroll
-    for (int j = 0; j < KC; j++)
-        xv[j] = (c != PAD && (uint32_t)j < kv) ? (AT)x[(size_t)c * ldx + j0 + j] : (AT)0;
-}
-
-// One pass over KC right-hand sides.  th == 1: four lanes per tile, the
-// reference's reduce tree per right-hand side; th > 1: two columns per lane,
-// fma chains over the brick-rows.  Normal blocks stage unit sums in LDS and
-// give each row to one lane; long rows keep per-lane sums.
 template <typename VT, typename XT, typename AT, int KC, bool H1>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
@@ -83,136 +28,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
     AT *__restrict__ y, uint32_t ldy, uint32_t j0, uint32_t kv, uint32_t th, uint32_t nrows,
     double *partial)
 {
-    __shared__ AT s_val[WAVES][G_UMAX * KC];
-    __shared__ uint32_t s_key[WAVES][G_UMAX];
-    __shared__ uint16_t s_seg[WAVES][G_UMAX + 1];
-    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    __shared__ GatherLds<AT, KC> lds[WAVES];
     const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
-    const uint32_t nu = d.uend - d.ubeg, q = lane & 3, C = 2 * q;
-    AT lacc[KC];
-#pragma unroll
-    for (int j = 0; j < KC; j++) lacc[j] = 0;
-    // tiles of this block; UG steps of 16 tiles are loaded before their
-    // gathers are issued, so a lane keeps 2*UG gathers in flight
-    constexpr int UG = KC >= 8 ? 1 : (KC >= 4 ? 2 : 4);
-    const uint32_t T0 = H1 ? d.ubeg : d.ubeg / 8, T1 = H1 ? d.uend : (d.uend + 7) / 8;
-    for (uint32_t Tbb = T0; Tbb < T1; Tbb += 16 * UG) {
-        AT a0[UG][KC], a1[UG][KC];
-#pragma unroll
-        for (int u = 0; u < UG; u++)
-#pragma unroll
-            for (int j = 0; j < KC; j++) a0[u][j] = a1[u][j] = 0;
-        for (uint32_t R = 0; R < th; R++) {
-            GLoad<VT> v0[UG], v1[UG];
-            u2v cc[UG];
-#pragma unroll
-            for (int u = 0; u < UG; u++) {
-                const uint32_t Tr = Tbb + 16 * u + lane / 4;
-                const size_t T = Tr < T1 ? Tr : T1 - 1;
-                const size_t sidx = (T * th + R) * 8 + C;
-                GPair<VT>::ld(aval + sidx, v0[u], v1[u]);
-                cc[u] = ntload(reinterpret_cast<const u2v *>(acol + sidx));
-            }
-#pragma unroll
-            for (int u = 0; u < UG; u++) {
-                AT x0[KC], x1[KC];
-                gather_x<XT, AT, KC>(x, cc[u][0], ldx, j0, kv, x0);
-                gather_x<XT, AT, KC>(x, cc[u][1], ldx, j0, kv, x1);
-#pragma unroll
-                for (int j = 0; j < KC; j++) {
-                    if (H1) {
-                        a0[u][j] = (AT)v0[u] * x0[j];
-                        a1[u][j] = (AT)v1[u] * x1[j];
-                    } else {
-                        a0[u][j] = gfma<AT>((AT)v0[u], x0[j], a0[u][j]);
-                        a1[u][j] = gfma<AT>((AT)v1[u], x1[j], a1[u][j]);
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < UG; u++) {
-            const uint32_t Tr = Tbb + 16 * u + lane / 4;
-            const size_t T = Tr < T1 ? Tr : T1 - 1;
-            if (H1) {
-                const bool ok = Tr < T1;
-                if (d.k != 0) {
-                    if (ok)
-#pragma unroll
-                        for (int j = 0; j < KC; j++) lacc[j] = lacc[j] + (a0[u][j] + a1[u][j]);
-                    continue;
-                }
-                const uint32_t key = ntload(sec + T);
-#pragma unroll
-                for (int j = 0; j < KC; j++) {
-                    AT p0 = a0[u][j], p1 = a1[u][j];
-                    p0 = p0 + dpp_any<AT, DPP_QUAD_XOR2>(p0);
-                    p1 = p1 + dpp_any<AT, DPP_QUAD_XOR2>(p1);
-                    p0 = p0 + dpp_any<AT, DPP_QUAD_XOR1>(p0);
-                    p1 = p1 + dpp_any<AT, DPP_QUAD_XOR1>(p1);
-                    if (ok && q == 0) s_val[w][(Tr - d.ubeg) * KC + j] = p0 + p1;
-                }
-                if (ok && q == 0) s_key[w][Tr - d.ubeg] = key;
-            } else {
-                const uint32_t u0 = 8 * Tr + C;
-                const bool ok0 = Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
-                const bool ok1 = Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
-                if (d.k != 0) {
-#pragma unroll
-                    for (int j = 0; j < KC; j++) {
-                        if (ok0) lacc[j] = lacc[j] + a0[u][j];
-                        if (ok1) lacc[j] = lacc[j] + a1[u][j];
-                    }
-                    continue;
-                }
-                const u2v kk = ntload(reinterpret_cast<const u2v *>(sec + 8 * T + C));
-#pragma unroll
-                for (int j = 0; j < KC; j++) {
-                    if (ok0) s_val[w][(u0 - d.ubeg) * KC + j] = a0[u][j];
-                    if (ok1) s_val[w][(u0 + 1 - d.ubeg) * KC + j] = a1[u][j];
-                }
-                if (ok0) s_key[w][u0 - d.ubeg] = kk[0];
-                if (ok1) s_key[w][u0 + 1 - d.ubeg] = kk[1];
-            }
-        }
-    }
-    if (d.k != 0) {
-#pragma unroll
-        for (int j = 0; j < KC; j++) {
-            AT t = wave_sum<AT>(lacc[j]);
-            if (lane == 0 && (uint32_t)j < kv) {
-                if (d.k == 1) {
-                    if (d.rbeg < nrows) y[(size_t)d.rbeg * ldy + j0 + j] = t;
-                } else {
-                    partial[(size_t)d.piece * PARTIAL_STRIDE + j0 + j] = (double)t;
-                }
-            }
-        }
-        return;
-    }
-    wave_lds_sync();
-    const uint32_t nseg = find_segments(s_key[w], s_seg[w], nu);
-    for (uint32_t sg = lane; sg < nseg; sg += WAVE) {
-        const uint32_t a = s_seg[w][sg], b = s_seg[w][sg + 1], r = s_key[w][a];
-        for (int j = 0; j < KC; j++) {
-            if ((uint32_t)j >= kv) break;
-            AT acc = 0;
-            for (uint32_t i = a; i < b; i++) acc = acc + s_val[w][i * KC + j];
-            if (r < nrows) y[(size_t)r * ldy + j0 + j] = acc;
-        }
-        // empty rows before this run (and after the last run) of the block
-        const uint32_t lo = (sg == 0) ? d.rbeg : s_key[w][s_seg[w][sg - 1]] + 1;
-        for (uint32_t z = lo; z < r && z < nrows; z++)
-            for (uint32_t j = 0; j < kv; j++) y[(size_t)z * ldy + j0 + j] = 0;
-        if (sg == nseg - 1)
-            for (uint32_t z = r + 1; z < d.rend && z < nrows; z++)
-                for (uint32_t j = 0; j < kv; j++) y[(size_t)z * ldy + j0 + j] = 0;
-    }
-    if (nseg == 0)
-        for (uint32_t z = d.rbeg + lane; z < d.rend && z < nrows; z += WAVE)
-            for (uint32_t j = 0; j < kv; j++) y[(size_t)z * ldy + j0 + j] = 0;
+    GatherLds<AT, KC> &L = lds[threadIdx.x / WAVE];
+    gather_block<VT, XT, AT, KC, H1>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv, th, nrows,
+                                     partial, L.val, L.key, L.seg);
 }
 
 // A_bar[s] = y_bar[row(s)] * x[acol[s]] (0 on padding).  row(s) is the sec
@@ -303,7 +125,7 @@ __global__ __launch_bounds__(256) void refresh_x_kernel(uint32_t nslots, uint32_
 template <typename VT, typename XT, typename AT>
 static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t *acol,
                          const uint32_t *sec, const void *x, uint32_t ldx, void *y, uint32_t ldy,
-                         uint32_t k)
+                         uint32_t k, uint32_t kcf)
 {
     const uint32_t nb = p->nblocks, th = p->axt.tileH;
     if (nb == 0) return INTSPMV_OK;
@@ -312,17 +134,18 @@ static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t
     const uint32_t nl = p->nlong_wg;
     for (uint32_t j0 = 0; j0 < k;) {
         const uint32_t rem = k - j0;
-        const uint32_t kc = rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1));
+        const uint32_t kc = kcf ? kcf : (rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1)));
+        const uint32_t kv = rem < kc ? rem : kc;
 #define ISPV_G(KC)                                                                              \
     if (kc == KC) {                                                                             \
         if (th == 1)                                                                            \
             axt_gather_kernel<VT, XT, AT, KC, true><<<grid, blk, 0, s>>>(                       \
                 p->d_blocks, nl, nb, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y,  \
-                ldy, j0, KC, th, p->nrows, p->d_partial);                                       \
+                ldy, j0, kv, th, p->nrows, p->d_partial);                                       \
         else                                                                                    \
             axt_gather_kernel<VT, XT, AT, KC, false><<<grid, blk, 0, s>>>(                      \
                 p->d_blocks, nl, nb, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y,  \
-                ldy, j0, KC, th, p->nrows, p->d_partial);                                       \
+                ldy, j0, kv, th, p->nrows, p->d_partial);                                       \
     }
         ISPV_G(1)
         ISPV_G(2)
@@ -348,9 +171,10 @@ static bool gather_plan_ok(const intspmv_ctx *ctx, const intspmv_plan *plan)
 
 extern "C" int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval,
                                const uint32_t *acol, const uint32_t *sec, const void *x,
-                               uint32_t ldx, void *y, uint32_t ldy, uint32_t k)
+                               uint32_t ldx, void *y, uint32_t ldy, uint32_t k, uint32_t kc)
 {
-    if (!gather_plan_ok(ctx, plan) || k == 0 || k > PARTIAL_STRIDE || ldx < k || ldy < k)
+    if (!gather_plan_ok(ctx, plan) || k == 0 || k > PARTIAL_STRIDE || ldx < k || ldy < k ||
+        (kc != 0 && kc != 1 && kc != 2 && kc != 4 && kc != 8))
         return INTSPMV_ERR_ARG;
     if (plan->nrows && (!x || !y || (plan->nunits && (!aval || !acol || !sec))))
         return INTSPMV_ERR_ARG;
@@ -358,11 +182,11 @@ extern "C" int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const
     if (st) return st;
     const intspmv_axt_desc &D = plan->axt;
     if (D.val_type == INTSPMV_F64 && D.x_type == INTSPMV_F64)
-        return launch_gather<double, double, double>(plan, aval, acol, sec, x, ldx, y, ldy, k);
+        return launch_gather<double, double, double>(plan, aval, acol, sec, x, ldx, y, ldy, k, kc);
     if (D.val_type == INTSPMV_F32 && D.x_type == INTSPMV_F32)
-        return launch_gather<float, float, float>(plan, aval, acol, sec, x, ldx, y, ldy, k);
+        return launch_gather<float, float, float>(plan, aval, acol, sec, x, ldx, y, ldy, k, kc);
     if (D.val_type == INTSPMV_BF16 && D.x_type == INTSPMV_F32)
-        return launch_gather<uint16_t, float, float>(plan, aval, acol, sec, x, ldx, y, ldy, k);
+        return launch_gather<uint16_t, float, float>(plan, aval, acol, sec, x, ldx, y, ldy, k, kc);
     return INTSPMV_ERR_UNSUPPORTED;
 }
 

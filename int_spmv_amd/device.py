@@ -70,11 +70,17 @@ SIGNATURES = [
     ("spmv_k1_order", C.c_int, [_vp, _u32, _vp, _vp, _vp]),
     ("intspmv_axt_plan", C.c_int, [_vp, C.POINTER(AxtDesc), _vp, C.POINTER(_vp)]),
     ("spmv_axt", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
-    ("spmv_axt_gather", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32]),
+    ("spmv_axt_gather", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32, _u32]),
     ("spmv_axt_grad_vals", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_axt_grad_vals_embedded", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_permute_vals", C.c_int, [_vp, _u32, C.c_int, _vp, _vp, _vp]),
     ("spmv_axt_refresh_x", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("intspmv_batch_create", C.c_int, [_vp, C.c_int, C.c_int, _u32, _vp, C.POINTER(_vp)]),
+    ("intspmv_batch_launch", C.c_int, [_vp, _vp]),
+    ("intspmv_batch_graph_capture", C.c_int, [_vp, _vp]),
+    ("intspmv_batch_graph_launch", C.c_int, [_vp, _vp]),
+    ("intspmv_batch_info", C.c_int, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
+    ("intspmv_batch_destroy", C.c_int, [_vp]),
     ("intspmv_plan_destroy", C.c_int, [_vp]),
     ("intspmv_plan_info", C.c_int, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
     ("intspmv_plan_layout", C.c_int, [_u32, _vp, _u32, _u32, _u32, _u32, _vp, C.c_size_t,
@@ -363,11 +369,12 @@ class DeviceAXTGather:
         self.plan = _Plan(eng, h)
 
     def matvec(self, x, y, k: int = 1, ldx: int | None = None, ldy: int | None = None,
-               vals=None) -> None:
-        """y = A x for k right-hand sides (x: n*ldx, y: n*ldy elements)."""
+               vals=None, kc: int = 0) -> None:
+        """y = A x for k right-hand sides (x: n*ldx, y: n*ldy elements); kc =
+        right-hand sides per pass (0: automatic)."""
         self.eng._check(lib().spmv_axt_gather(
             self.eng.h, self.plan.h, _dptr(vals if vals is not None else self.aval),
-            _dptr(self.acol), _dptr(self.sec), _dptr(x), ldx or k, _dptr(y), ldy or k, k),
+            _dptr(self.acol), _dptr(self.sec), _dptr(x), ldx or k, _dptr(y), ldy or k, k, kc),
             "spmv_axt_gather")
 
     def grad_vals(self, x, ybar, vbar) -> None:
@@ -428,3 +435,58 @@ def grad_vals_embedded(dev_axt: "DeviceAXT", ybar, vbar) -> None:
                                                          _dptr(dev_axt.ax), _dptr(dev_axt.sec),
                                                          _dptr(ybar), _dptr(vbar)),
                        "spmv_axt_grad_vals_embedded")
+
+
+class BatchMatrix(C.Structure):
+    _fields_ = [("aval", _vp), ("acol", _vp), ("sec", _vp), ("x", _vp), ("y", _vp),
+                ("ldx", _u32), ("ldy", _u32), ("k", _u32), ("kc", _u32), ("nrows", _u32),
+                ("tileH", _u32), ("tileN", _u32), ("lenAX", _u32), ("colN", _u32),
+                ("umax", _u32), ("con_host", _vp)]
+
+
+class DeviceBatch:
+    """Many column-map AXT matrices with their own tile heights and 1..32
+    right-hand sides in one persistent launch (optionally a hipGraph).
+
+    entries: list of dicts with keys a (host AXT built with want_acol), X and
+    Y (device tensors, n*k row-major), k and optionally kc."""
+
+    def __init__(self, eng: Engine, entries, val_type: int = F32, x_type: int = F32):
+        self.eng = eng
+        self._keep = []
+        arr = (BatchMatrix * len(entries))()
+        for i, e in enumerate(entries):
+            a = e["a"]
+            aval = eng.upload(_value_plane(a, val_type))
+            acol, sec = eng.upload(a.acol), eng.upload(a.sec)
+            con = np.ascontiguousarray(a.con)
+            self._keep += [aval, acol, sec, con, e["X"], e["Y"]]
+            k = e["k"]
+            arr[i] = BatchMatrix(aval=aval.data_ptr(), acol=acol.data_ptr(), sec=sec.data_ptr(),
+                                 x=e["X"].data_ptr(), y=e["Y"].data_ptr(), ldx=k, ldy=k, k=k,
+                                 kc=e.get("kc", 0), nrows=a.nrows, tileH=a.tileH, tileN=a.tileN,
+                                 lenAX=a.lenAX, colN=a.colN, umax=e.get("umax", 0),
+                                 con_host=con.ctypes.data)
+        h = _vp()
+        eng._check(lib().intspmv_batch_create(eng.h, val_type, x_type, len(entries), arr,
+                                              C.byref(h)), "batch_create")
+        self.h = h
+
+    def info(self) -> dict:
+        a, b, c = _u32(), _u32(), _u32()
+        lib().intspmv_batch_info(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return dict(nitems=a.value, grid=b.value, nsplit=c.value)
+
+    def launch(self) -> None:
+        self.eng._check(lib().intspmv_batch_launch(self.eng.h, self.h), "batch_launch")
+
+    def capture(self) -> None:
+        self.eng._check(lib().intspmv_batch_graph_capture(self.eng.h, self.h), "batch_capture")
+
+    def replay(self) -> None:
+        self.eng._check(lib().intspmv_batch_graph_launch(self.eng.h, self.h), "batch_replay")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.intspmv_batch_destroy(self.h)
+            self.h = None

@@ -61,6 +61,11 @@ class _AXT(C.Structure):
                 ("nzslot", u32p)]
 
 
+class TuneChoice(C.Structure):
+    _fields_ = [("tileH", C.c_uint32), ("kc", C.c_uint32), ("slots", C.c_uint64),
+                ("bytes", C.c_uint64)]
+
+
 _lib = None
 
 
@@ -110,6 +115,8 @@ def lib():
         L.intspmv_gen_banded.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
                                          C.c_uint32, C.c_uint64, C.POINTER(_CSR)]
         L.intspmv_wtime.restype = C.c_double
+        L.intspmv_tune_axt.argtypes = [C.POINTER(_CSR), C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.POINTER(TuneChoice), C.c_uint32, u32p]
         _lib = L
     return _lib
 
@@ -378,6 +385,14 @@ class AXT:
     def value_plane(self) -> np.ndarray:
         """Stored values in slot order (slot = T*tileH*8 + R*8 + C)."""
         return np.ascontiguousarray(self.ax.reshape(-1, 2, 8)[:, 0, :]).reshape(-1)
+
+
+def tune_model(m: CSR, k: int, sv: int = 8, sx: int = 8) -> list[dict]:
+    """Candidate (tileH, kc) layouts ranked by modelled bytes (intspmv_tune_axt)."""
+    out = (TuneChoice * 20)()
+    n = C.c_uint32()
+    _check(lib().intspmv_tune_axt(C.byref(m._s), k, sv, sx, out, 20, C.byref(n)), "tune")
+    return [dict(tileH=o.tileH, kc=o.kc, slots=o.slots, bytes=o.bytes) for o in out[: n.value]]
 
 
 def wtime() -> float:

@@ -39,6 +39,21 @@ def rmat_22(seed: int = 0):
     return m, H.uniform(m.nrows, -1.0, 1.0, seed=seed)
 
 
+def batch_256(seed: int = 0, count: int = 256, nmax: int = 200_000):
+    """Many matrices: n log-uniform in [1000, nmax], 4..64 entries per row,
+    70% of columns in a band of width 0.05 n, 30% uniform, values N(0,1);
+    k in {1, 4, 8, 16, 32} right-hand sides each.  Returns [(m, k)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    ks = (1, 4, 8, 16, 32)
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(nmax))))
+        m = H.CSR.banded(n, 4, 64, band_frac=0.7, band_w=max(2, int(0.05 * n)), seed=seed * 1000003 + i)
+        out.append((m, ks[int(rng.integers(0, len(ks)))]))
+    return out
+
+
 WORKLOADS = {
     "laplace_256": laplace_256,
     "powerlaw_1m": powerlaw_1m,

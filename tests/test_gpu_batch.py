@@ -1,0 +1,88 @@
+"""GPU: the batched persistent launch (many matrices, uneven sizes, own tile
+heights, 1..32 right-hand sides), its hipGraph capture and replay, and the
+autotuner."""
+import numpy as np
+import pytest
+
+import oracle as O
+from int_spmv_amd import device as D
+from int_spmv_amd import host as H
+from int_spmv_amd.workloads import batch_256
+
+from .util import bitwise_equal, csr_long_rows, scaled
+
+pytestmark = pytest.mark.gpu
+
+
+def _entries(engine, mats, ths, dtype):
+    import torch
+    ents = []
+    for i, (m, k) in enumerate(mats):
+        n = m.nrows
+        X = np.stack([H.uniform(n, seed=50 + i * 40 + j) for j in range(k)], axis=1)
+        if dtype == torch.float32:
+            X = X.astype(np.float32).astype(np.float64)
+        Xd = torch.tensor(X.reshape(-1), dtype=dtype, device=engine.torch_device)
+        Yd = torch.full((n * k,), float("nan"), dtype=dtype, device=engine.torch_device)
+        a = H.AXT(m, np.zeros(n), th=ths[i % len(ths)], want_acol=True)
+        ents.append(dict(a=a, X=Xd, Y=Yd, k=k, Xh=X, m=m))
+    return ents
+
+
+def _y(e):
+    import torch
+    torch.cuda.synchronize()
+    return e["Y"].double().cpu().numpy().reshape(e["m"].nrows, e["k"])
+
+
+def test_batch_f64_equals_single(engine):
+    import torch
+    mats = [(m, k) for (m, k) in batch_256(seed=1, count=10, nmax=20000)]
+    mats.append((csr_long_rows(), 3))  # split rows inside a batch
+    ents = _entries(engine, mats, (1, 2, 4, 8), torch.float64)
+    b = D.DeviceBatch(engine, ents, D.F64, D.F64)
+    b.launch()
+    assert b.info()["nsplit"] > 0
+    for e in ents:
+        g = D.DeviceAXTGather(engine, e["a"], D.F64, D.F64)
+        Y1 = torch.empty_like(e["Y"])
+        g.matvec(e["X"], Y1, k=e["k"])
+        torch.cuda.synchronize()
+        Yb = _y(e)
+        assert bitwise_equal(Yb.reshape(-1), Y1.cpu().numpy()), e["m"].nrows
+        m = e["m"]
+        for j in range(e["k"]):
+            yr = O.cf_csr(m.row, m.col, m.val, e["Xh"][:, j])
+            assert scaled(m, e["Xh"][:, j], yr, Yb[:, j]) <= 1e-12
+
+
+def test_batch_f32_and_graph_replay(engine):
+    import torch
+    mats = batch_256(seed=2, count=24, nmax=30000)
+    ents = _entries(engine, mats, (1, 4), torch.float32)
+    b = D.DeviceBatch(engine, ents, D.F32, D.F32)
+    b.launch()
+    eager = [_y(e).copy() for e in ents]
+    for e in ents:
+        m = e["m"]
+        v32 = m.val.astype(np.float32).astype(np.float64)
+        for j in range(e["k"]):
+            yr = O.cf_csr(m.row, m.col, v32, e["Xh"][:, j])
+            assert scaled(m, e["Xh"][:, j], yr, eager[ents.index(e)][:, j]) <= 1e-6
+    b.capture()
+    for rep in range(3):
+        for e in ents:
+            e["Y"].fill_(float("nan"))       # eager work between replays
+        torch.cuda.synchronize()
+        b.replay()
+        for i, e in enumerate(ents):
+            assert bitwise_equal(_y(e).reshape(-1), eager[i].reshape(-1)), (rep, i)
+
+
+def test_autotuner_picks_a_measured_candidate(engine):
+    from int_spmv_amd.tune import tune_axt
+    m = H.CSR.banded(30000, 4, 64, band_frac=0.7, band_w=1500, seed=3)
+    best = tune_axt(engine, m, 8, D.F32, D.F32, top=3, reps=3)
+    assert best["tileH"] in (1, 2, 4, 8, 16) and best["kc"] in (1, 2, 4, 8)
+    assert best["us"] > 0 and len(best["ranked"]) == 3
+    assert (best["tileH"], best["kc"]) in [(c["tileH"], c["kc"]) for c in best["ranked"]]
