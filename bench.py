@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--workload", default="powerlaw_1m")
     p.add_argument("--quick", action="store_true", help="headline kernel only (profiling)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--batch-count", type=int, default=256,
+                   help="matrices in the batched-launch measurement (0 = skip)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run a plain device copy of this many MiB (counter calibration)")
     return p.parse_args()
@@ -63,6 +65,55 @@ def timed(fn, steps, warmup, torch):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     return (t1 - t0) / steps, e0.elapsed_time(e1) * 1e-3 / steps
+
+
+def bench_batch(eng, count, torch, log):
+    """256 matrices (n log-uniform in [1e3, 2e5], 4..64 entries per row, 70 %
+    banded), FP32 values and x, k in {1,4,8,16,32} each; per-matrix tile height
+    and right-hand-side blocking from the autotuner's byte model; one
+    persistent launch captured in a hipGraph and replayed."""
+    import numpy as np
+
+    from int_spmv_amd import device as D
+    from int_spmv_amd import host as H
+    from int_spmv_amd import metrics as M
+    from int_spmv_amd.workloads import batch_256
+    t0 = time.perf_counter()
+    mats = batch_256(seed=0, count=count)
+    ents, flops, bytes_, rows, nnz = [], 0.0, 0, 0, 0
+    for i, (m, k) in enumerate(mats):
+        c = H.tune_model(m, k, 4, 4)[0]
+        a = H.AXT(m, np.zeros(m.nrows), th=c["tileH"], want_acol=True)
+        X = torch.rand(m.nrows * k, dtype=torch.float32, device=eng.torch_device)
+        Y = torch.empty(m.nrows * k, dtype=torch.float32, device=eng.torch_device)
+        ents.append(dict(a=a, X=X, Y=Y, k=k, kc=c["kc"]))
+        flops += M.flops(m.nnz, k)
+        bytes_ += M.bytes_axt_gather(m.nrows, a.lenAX, a.lenSEC, k=k, sv=4, sx=4, sy=4)
+        rows += m.nrows
+        nnz += m.nnz
+    b = D.DeviceBatch(eng, ents, D.F32, D.F32)
+    for e in ents:
+        e.pop("a")
+    info = b.info()
+    setup = time.perf_counter() - t0
+    b.capture()
+    for _ in range(3):
+        b.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        b.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    r = {"matrices": count, "rows": rows, "nnz": nnz, "us": round(t * 1e6, 1),
+         "GFLOPs": round(flops / t / 1e9, 1), "GBs": round(bytes_ / t / 1e9, 1),
+         "pct_hbm": round(100 * bytes_ / t / 1e9 / M.HBM_PEAK_GBS, 1), "items": info["nitems"],
+         "grid": info["grid"], "setup_s": round(setup, 1)}
+    log(f"[bench] BATCH: {r}")
+    return r
 
 
 def read_traffic(name: str):
@@ -195,6 +246,10 @@ def main():
         record("K1P_F64", M.bytes_k1(n, k.lenVC, k.chunkNum), w, e)
         del dk, k
 
+    batch = None
+    if not args.quick and args.batch_count > 0:
+        batch = bench_batch(eng, args.batch_count, torch, log)
+
     # ---- CPU baseline + correctness (oracle as the checker) -------------------
     cpu = None
     if not args.no_cpu:
@@ -262,6 +317,7 @@ def main():
                      "kernel": "axt_h1_kernel<double,double,double,false> (+ split fix-up)"},
         "cpu_baseline": cpu,
         "formats": formats,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

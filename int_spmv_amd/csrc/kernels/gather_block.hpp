@@ -49,11 +49,35 @@ __device__ __forceinline__ double gfma<double>(double a, double b, double c) { r
 template <>
 __device__ __forceinline__ float gfma<float>(float a, float b, float c) { return fmaf(a, b, c); }
 
-// x values of one slot for KC right-hand sides starting at j0 (0 for padding)
+// x values of one slot for KC right-hand sides starting at j0 (0 for padding).
+// When the row offset c*ldx + j0 is a multiple of the vector width the KC
+// values come in 16-byte loads.
 template <typename XT, typename AT, int KC>
 __device__ __forceinline__ void gather_x(const XT *x, uint32_t c, uint32_t ldx, uint32_t j0,
                                          uint32_t kv, AT (&xv)[KC])
 {
+    constexpr int VW = 16 / sizeof(XT); // elements per 16-byte load
+    if constexpr (KC >= VW) {
+        if (c != PAD && kv == (uint32_t)KC && ((ldx | j0) % VW) == 0) {
+            const XT *p = x + (size_t)c * ldx + j0;
+#pragma unroll
+            for (int j = 0; j < KC; j += VW) {
+                if constexpr (sizeof(XT) == 8) {
+                    d2v v = *reinterpret_cast<const d2v *>(p + j);
+                    xv[j] = (AT)v[0];
+                    xv[j + 1] = (AT)v[1];
+                } else {
+                    typedef float f4v __attribute__((ext_vector_type(4)));
+                    f4v v = *reinterpret_cast<const f4v *>(p + j);
+                    xv[j] = (AT)v[0];
+                    xv[j + 1] = (AT)v[1];
+                    xv[j + 2] = (AT)v[2];
+                    xv[j + 3] = (AT)v[3];
+                }
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < KC; j++)
         xv[j] = (c != PAD && (uint32_t)j < kv) ? (AT)x[(size_t)c * ldx + j0 + j] : (AT)0;
