@@ -67,6 +67,24 @@ def timed(fn, steps, warmup, torch):
     return (t1 - t0) / steps, e0.elapsed_time(e1) * 1e-3 / steps
 
 
+def timed_each(fn, steps, warmup, torch, between=None):
+    """Per-launch HIP-event times (seconds), each launch bracketed by its own
+    events on the current stream; `between` (untimed) runs before each launch."""
+    for _ in range(warmup):
+        fn()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    torch.cuda.synchronize()
+    for a, b in evs:
+        if between is not None:
+            between()
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    return sorted(a.elapsed_time(b) * 1e-3 for a, b in evs)
+
+
 def bench_batch(eng, count, torch, log):
     """256 matrices (n log-uniform in [1e3, 2e5], 4..64 entries per row, 70 %
     banded), FP32 values and x, k in {1,4,8,16,32} each; per-matrix tile height
@@ -116,6 +134,111 @@ def bench_batch(eng, count, torch, log):
     return r
 
 
+def rocsparse_leg(dc, xd, y_csr, n, nnz):
+    """rocSPARSE csrmv (vendor comparator, run and timed only) on the same
+    device CSR arrays as spmv_csr; each algorithm's result is compared with
+    the ordered product already in y_csr (max |diff| / max |y|, as getErrors)."""
+    import torch
+
+    from int_spmv_amd import comparators as CMP
+    if not CMP.rocsparse_available():
+        return None
+    ref = y_csr[:n].clone()
+    yr = torch.empty_like(ref)
+    out = {}
+    for name, alg in CMP.ROCSPARSE_CSR_ALGS.items():
+        yr.zero_()  # rocSPARSE forms beta*y even for beta = 0 in some algorithms
+        r = CMP.rocsparse_csr(dc.row, dc.col, dc.val, xd[:n], yr, n, nnz, torch,
+                              algs={name: alg})[name]
+        if isinstance(r, float):
+            rel = float((yr - ref).abs().max() / ref.abs().max().clamp_min(1e-300))
+            out[name] = (r, rel)
+        else:
+            out[name] = (None, r)
+    return out
+
+
+def cpu_leg(m, x, a1, hosts, formats, y1, head_bytes, workload, quick):
+    """Errors of the GPU results against the FP64 oracle, and the CPU baseline:
+    the oracle's OpenMP restatements of the reference kernels (and MKL's CSR
+    kernels when libmkl_rt is present) timed on this host's cores."""
+    import ctypes as C
+
+    import numpy as np
+
+    import oracle as O
+    from int_spmv_amd import comparators as CMP
+    from int_spmv_amd import metrics as M
+    n, nnz = m.nrows, m.nnz
+    yref = O.cf_csr(m.row, m.col, m.val, x)
+
+    def serr(y):
+        return O.scaled_err(m.row, m.col, m.val, x, yref, y)[0]
+    formats["AXT_NOC_H1_F64"]["max_scaled_err_vs_csr"] = serr(y1[:n].cpu().numpy())
+    for nm in ("AXT_NOC_H1_F32", "AXT_NOC_H1_BF16"):
+        if nm in formats and "_y" in formats[nm]:
+            yy = formats[nm].pop("_y")[:n].double().cpu().numpy()
+            formats[nm]["max_scaled_err_vs_fp64"] = serr(yy)
+    threads = cores()
+    L = O.lib()
+
+    def run(kind, thr, budget, len_, th, arrs):
+        keep = [np.ascontiguousarray(a) for a in arrs]
+        ptrs = [C.c_void_p(a.ctypes.data) for a in keep] + [None] * (6 - len(keep))
+        y = np.zeros(n)
+        t1 = L.orc_bench(kind, thr, 1, n, len_, th, *ptrs, C.c_void_p(y.ctypes.data))
+        if t1 < 0:
+            raise RuntimeError(f"orc_bench kind {kind}")
+        iters = int(max(1, min(250, budget / max(t1, 1e-6))))
+        t = L.orc_bench(kind, thr, iters, n, len_, th, *ptrs, C.c_void_p(y.ctypes.data))
+        return t, iters, y
+
+    t, iters, ycpu = run(4, threads, 10.0, a1.tileN, 1, [a1.ax, a1.sec])
+    cpu = {"value": round(head_bytes / t / 1e9, 2), "unit": "GB/s", "cores": threads,
+           "kind": "port",
+           "sample": f"int_axt_noc_h1 OpenMP restatement (dynamic schedule, atomic y), "
+                     f"full {workload} matrix, {iters} calls, y zeroed untimed; "
+                     f"{t * 1e3:.2f} ms/call",
+           "max_scaled_err_vs_csr": serr(ycpu)}
+    log(f"[bench] cpu baseline: {cpu}")
+    if quick:
+        return cpu
+    others = {}
+
+    def rec(name, t, calls, thr, bytes_, y, kind="port"):
+        others[name] = {"ms": round(t * 1e3, 3), "GBs": round(bytes_ / t / 1e9, 2),
+                        "GFLOPs": round(M.flops(nnz) / t / 1e9, 2), "calls": calls,
+                        "threads": thr, "kind": kind, "max_scaled_err_vs_csr": serr(y)}
+        log(f"[bench] cpu {name}: {others[name]}")
+    t, it, y = run(0, 1, 2.0, 0, 0, [m.row, m.col, m.val, x])
+    rec("cf_CSR", t, it, 1, M.bytes_csr(n, nnz), y)
+    if "AXC" in hosts:
+        c = hosts["AXC"]
+        t, it, y = run(1, threads, 2.0, 0, 0, [c.ax, c.brp])
+        rec("int_axc", t, it, threads, M.bytes_axc(n, c.lenAX), y)
+    if "K1" in hosts:
+        k = hosts["K1"]
+        arrs = [k.val, k.col, k.chp, k.nmc, x, k.permi]
+        t, it, y = run(2, threads, 2.0, k.chunkNum, 0, arrs)
+        rec("int_k1 (+orderArrayK1 untimed)", t, it, threads, M.bytes_k1(n, k.lenVC, k.chunkNum), y)
+        t, it, y = run(3, threads, 2.0, k.chunkNum, 0, arrs)
+        rec("int_k1p", t, it, threads, M.bytes_k1(n, k.lenVC, k.chunkNum), y)
+    if "AXT_H4" in hosts:
+        a4 = hosts["AXT_H4"]
+        t, it, y = run(5, threads, 2.0, a4.tileN, 4, [a4.ax, a4.sec])
+        rec("int_axt_noc th4", t, it, threads, M.bytes_axt(n, a4.lenAX, a4.lenSEC), y)
+    if CMP.mkl_available():
+        for thr in sorted({1, threads}):
+            res = CMP.mkl_csr(m.row, m.col, m.val, x, thr, budget_s=1.5)
+            for nm, r in (res or {}).items():
+                rec(f"{nm} ({thr} thr)", r["ms"] * 1e-3, r["calls"], thr, M.bytes_csr(n, nnz),
+                    r["y"], kind="vendor")
+    else:
+        others["mkl"] = "libmkl_rt not found"
+    cpu["formats"] = others
+    return cpu
+
+
 def read_traffic(name: str):
     """HBM bytes per launch from a committed rocprofv3 PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", name)
@@ -132,8 +255,8 @@ def read_traffic(name: str):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (every box here "
-              f"has a single MI355X).", file=sys.stderr)
+        print(f"bench.py measures the SpMV path on one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is designed but not built).", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -183,6 +306,18 @@ def main():
     record("AXT_NOC_H1_F64", head_bytes, wall, ev,
            {"beta": round(a1.fd.beta, 4), "plan": d1.plan.info()})
     head_wall, head_ev = wall, ev
+    # per-launch events: median warm, and cold (a 1 GiB buffer written between launches
+    # evicts the matrix from the 256 MB MALL and the L2s)
+    warm = timed_each(lambda: d1.int_axt(y1), args.steps, 1, torch)
+    scratch = torch.empty(1 << 27, dtype=torch.float64, device=eng.torch_device)
+    cold = timed_each(lambda: d1.int_axt(y1), args.steps, 1, torch, between=lambda: scratch.fill_(1.0))
+    del scratch
+    med, cmed = warm[len(warm) // 2], cold[len(cold) // 2]
+    formats["AXT_NOC_H1_F64"].update({
+        "us_median": round(med * 1e6, 2), "GBs_median": round(head_bytes / med / 1e9, 1),
+        "us_cold_median": round(cmed * 1e6, 2), "GBs_cold": round(head_bytes / cmed / 1e9, 1)})
+    log(f"[bench] AXT_NOC_H1_F64 per-launch median {med * 1e6:.2f} us, cold {cmed * 1e6:.2f} us")
+    hosts = {}
 
     if not args.quick:
         for th in (2, 4, 8, 16):
@@ -192,6 +327,8 @@ def main():
             w, e = timed(lambda: d.int_axt(y), 20, 3, torch)
             record(f"AXT_NOC_H{th}_F64", M.bytes_axt(n, a.lenAX, a.lenSEC), w, e,
                    {"beta": round(a.fd.beta, 4)})
+            if th == 4:
+                hosts["AXT_H4"] = a
             del d, a, y
         for vt, xt, nm, sv, sx in ((D.F32, D.F32, "AXT_NOC_H1_F32", 4, 4),
                                   (D.BF16, D.F32, "AXT_NOC_H1_BF16", 2, 4)):
@@ -234,17 +371,28 @@ def main():
         record("CSR_F64", M.bytes_csr(n, nnz), w, e)
         w, e = timed(lambda: dc.cf_csr(xd, y), 3, 1, torch)
         record("CSR_ORDERED_F64", M.bytes_csr(n, nnz), w, e)
+        rs = rocsparse_leg(dc, xd, y, n, nnz)
+        if rs is None:
+            formats["ROCSPARSE_CSR_F64"] = "librocsparse not found"
+        for alg, (t, rel) in (rs or {}).items():
+            if t is None:
+                formats[f"ROCSPARSE_CSR_F64_{alg}"] = rel
+            else:
+                record(f"ROCSPARSE_CSR_F64_{alg}", M.bytes_csr(n, nnz), t, t,
+                       {"rel_err_vs_csr": rel})
         del dc
         c = H.AXC(m, x)
         dcc = D.DeviceAXC(eng, c)
         w, e = timed(lambda: dcc.int_axc(y), 20, 3, torch)
         record("AXC_F64", M.bytes_axc(n, c.lenAX), w, e)
-        del dcc, c
+        del dcc
+        hosts["AXC"] = c
         k = H.K1(m)
         dk = D.DeviceK1(eng, k)
         w, e = timed(lambda: dk.int_k1p(xd, y), 20, 3, torch)
         record("K1P_F64", M.bytes_k1(n, k.lenVC, k.chunkNum), w, e)
-        del dk, k
+        del dk
+        hosts["K1"] = k
 
     batch = None
     if not args.quick and args.batch_count > 0:
@@ -253,43 +401,10 @@ def main():
     # ---- CPU baseline + correctness (oracle as the checker) -------------------
     cpu = None
     if not args.no_cpu:
-        import numpy as np
-
-        import oracle as O
-        torch.cuda.synchronize()
-        yg = y1[:n].cpu().numpy()
-        yref = O.cf_csr(m.row, m.col, m.val, x)
-        err, _ = O.scaled_err(m.row, m.col, m.val, x, yref, yg)
-        formats["AXT_NOC_H1_F64"]["max_scaled_err_vs_csr"] = err
-        for nm in ("AXT_NOC_H1_F32", "AXT_NOC_H1_BF16"):
-            if nm in formats:
-                yy = formats[nm].pop("_y")[:n].double().cpu().numpy()
-                formats[nm]["max_scaled_err_vs_fp64"] = O.scaled_err(m.row, m.col, m.val, x,
-                                                                     yref, yy)[0]
-        threads = cores()
-        ycpu = np.zeros(n)
-        import ctypes as C
-        L = O.lib()
-
-        def p(a):
-            return C.c_void_p(a.ctypes.data)
-        ax = np.ascontiguousarray(a1.ax)
-        sec = np.ascontiguousarray(a1.sec)
-        t1 = L.orc_bench(4, threads, 1, n, a1.tileN, 1, p(ax), p(sec), None, None, None, None,
-                         p(ycpu))
-        iters = int(max(1, min(250, 10.0 / max(t1, 1e-6))))
-        tc = L.orc_bench(4, threads, iters, n, a1.tileN, 1, p(ax), p(sec), None, None, None,
-                         None, p(ycpu))
-        cpu = {"value": round(head_bytes / tc / 1e9, 2), "unit": "GB/s", "cores": threads,
-               "kind": "port",
-               "sample": f"int_axt_noc_h1 OpenMP restatement (dynamic schedule, atomic y), "
-                         f"full {args.workload} matrix, {iters} calls, y zeroed untimed; "
-                         f"{tc * 1e3:.2f} ms/call"}
-        cerr, _ = O.scaled_err(m.row, m.col, m.val, x, yref, ycpu)
-        cpu["max_scaled_err_vs_csr"] = cerr
-        log(f"[bench] cpu baseline: {cpu}")
+        cpu = cpu_leg(m, x, a1, hosts, formats, y1, head_bytes, args.workload, args.quick)
     for f in formats.values():
-        f.pop("_y", None)
+        if isinstance(f, dict):
+            f.pop("_y", None)
 
     achieved = head_bytes / head_ev / 1e9
     traffic = read_traffic(f"axt_h1_f64_{args.workload}_traffic.txt")
