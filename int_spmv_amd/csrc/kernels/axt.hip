@@ -27,6 +27,7 @@ namespace ispv {
 #endif
 constexpr uint32_t AXT_UMAX = ISPV_AXT_UMAX; // units (tiles or columns) per block
 constexpr uint32_t AXT_RMAX = 256;  // rows per block
+static_assert(AXT_RMAX <= BLOCK_ROWS_MAX, "keyed blocks map their rows in a 256-bit LDS map");
 // A long row is streamed by one wave up to this many slots (128 KB of FP64
 // interleaved tiles); longer rows are split across waves.
 #ifndef ISPV_LONG_SLOTS
@@ -254,6 +255,25 @@ __device__ __forceinline__ AT h1_long(const VT *av, const XT *xb, const BlockDes
     return wave_sum<AT>(acc.value());
 }
 
+// Stages a normal block's sec words (th = 1: one per tile) in LDS with one
+// lane per tile, so the keys cost two coalesced loads per block and no
+// registers across the value loads.
+__device__ __forceinline__ void stage_keys(const uint32_t *sec, const BlockDesc &d, uint32_t nu,
+                                           uint32_t *sk)
+{
+    if (nu == 0) return;
+    const int lane = lane_id();
+    uint32_t kk[AXT_UMAX / WAVE];
+#pragma unroll
+    for (int j = 0; j < (int)(AXT_UMAX / WAVE); j++) {
+        const uint32_t t = j * WAVE + lane;
+        kk[j] = ntload(sec + d.ubeg + (t < nu ? t : nu - 1));
+    }
+#pragma unroll
+    for (int j = 0; j < (int)(AXT_UMAX / WAVE); j++)
+        if (j * WAVE + lane < nu) sk[j * WAVE + lane] = kk[j];
+}
+
 #ifdef ISPV_WAVES_PER_EU
 #define ISPV_OCC __attribute__((amdgpu_waves_per_eu(ISPV_WAVES_PER_EU)))
 #else
@@ -285,15 +305,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
         // the x pair to the value lanes, whose quads then form the tree.
         const uint32_t p = lane & 7;
         constexpr int U = 2 * ISPV_H1_STEPS;
+        stage_keys(sec, d, nu, s_key[w]);
         for (uint32_t base = 0; base < nu; base += 8 * U) {
             d2v v[U];
-            uint32_t key[U];
 #pragma unroll
             for (int j = 0; j < U; j++) {
                 uint32_t t = base + j * 8 + lane / 8;
                 size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
                 v[j] = ntload(reinterpret_cast<const d2v *>(av + T * 16 + 2 * p));
-                key[j] = ntload(sec + T);
             }
 #pragma unroll
             for (int j = 0; j < U; j++) {
@@ -305,10 +324,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
                 p0 = p0 + dpp_f64<DPP_QUAD_XOR1>(p0);
                 p1 = p1 + dpp_f64<DPP_QUAD_XOR1>(p1);
                 uint32_t t = base + j * 8 + lane / 8;
-                if (t < nu && p == 0) {
-                    s_val[w][t] = (AT)(p0 + p1);
-                    s_key[w][t] = key[j];
-                }
+                if (t < nu && p == 0) s_val[w][t] = (AT)(p0 + p1);
             }
         }
         wave_lds_sync();
@@ -319,15 +335,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
         // FP32 interleaved, two 64-byte tiles per 128-byte line (see h1_long)
         const uint32_t p = lane & 7;
         constexpr int U = 2 * ISPV_H1_STEPS;
+        stage_keys(sec, d, nu, s_key[w]);
         for (uint32_t base = 0; base < nu; base += 16 * U) {
             float4 v[U];
-            uint32_t key[U];
 #pragma unroll
             for (int j = 0; j < U; j++) {
                 uint32_t t = base + j * 16 + 2 * (lane / 8) + p / 4;
                 size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
                 v[j] = f4load(reinterpret_cast<const float *>(av) + T * 16 + 4 * (p % 4));
-                key[j] = ntload(sec + T);
             }
 #pragma unroll
             for (int j = 0; j < U; j++) {
@@ -335,10 +350,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
                 float4 xx = f4dpp<DPP_ROW_SHL2>(v[j]);
                 float ps = (v[j].x * xx.x + v[j].y * xx.y) + (v[j].z * xx.z + v[j].w * xx.w);
                 ps = ps + dpp_f32<DPP_QUAD_XOR1>(ps); // values 0-3 + values 4-7
-                if (t < nu && p % 4 == 0) {
-                    s_val[w][t] = (AT)ps;
-                    s_key[w][t] = key[j];
-                }
+                if (t < nu && p % 4 == 0) s_val[w][t] = (AT)ps;
             }
         }
         wave_lds_sync();
@@ -348,23 +360,19 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
     if constexpr (PLANAR) {
         // planar storage: one lane per tile
         constexpr int U = ISPV_H1_STEPS;
+        stage_keys(sec, d, nu, s_key[w]);
         for (uint32_t base = 0; base < nu; base += 64 * U) {
             AT ts[U];
-            uint32_t key[U];
 #pragma unroll
             for (int j = 0; j < U; j++) {
                 uint32_t t = base + j * 64 + lane;
                 size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
                 ts[j] = planar_tile<VT, XT, AT>(av, xb, T);
-                key[j] = ntload(sec + T);
             }
 #pragma unroll
             for (int j = 0; j < U; j++) {
                 uint32_t t = base + j * 64 + lane;
-                if (t < nu) {
-                    s_val[w][t] = ts[j];
-                    s_key[w][t] = key[j];
-                }
+                if (t < nu) s_val[w][t] = ts[j];
             }
         }
         wave_lds_sync();
@@ -566,10 +574,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
         AT a = 0;
         for (uint32_t i = s_seg[w][s], b = s_seg[w][s + 1]; i < b; i++)
             a = fma_acc<AT>(s_v[w][i], s_x[w][i], a);
-        store_run<AT>(a, s, nseg, s_key[w], s_seg[w], d, y, nrows);
+        const uint32_t r = s_key[w][s_seg[w][s]];
+        if (r < nrows) y[r] = a;
     }
-    if (nseg == 0)
-        for (uint32_t z = d.rbeg + lane; z < d.rend && z < nrows; z += WAVE) y[z] = 0;
+    zero_empty_rows(s_key[w], s_seg[w], nseg, d, nrows, [&](uint32_t z) { y[z] = (AT)0; });
 }
 
 // th > 1, FP64 reference layout, full-line mapping: lane (tile Tb + l/8,

@@ -158,7 +158,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
         std::vector<BlockDesc> mb;
         std::vector<SplitDesc> ms;
         uint32_t nlw = 0, nlong = 0, npieces = 0;
-        int st = ispv_build_blocks(M.nrows, uptr.data(), umax, 256, long_units,
+        int st = ispv_build_blocks(M.nrows, uptr.data(), umax, BLOCK_ROWS_MAX, long_units,
                                    th == 1 ? 1024 : 8192 / th, mb, ms, nlw, nlong, npieces);
         if (st) return st;
         hm[i] = {M.aval, M.acol, M.sec, M.x, M.y, M.nrows, th, M.k, M.ldx, M.ldy, kc, piece_base, 0};

@@ -218,17 +218,10 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
             for (uint32_t i = a; i < b; i++) acc = acc + sv[i * KC + j];
             if (r < nrows) y[(size_t)r * ldy + j0 + j] = acc;
         }
-        // empty rows before this run (and after the last run) of the block
-        const uint32_t lo = (si == 0) ? d.rbeg : sk[sg[si - 1]] + 1;
-        for (uint32_t z = lo; z < r && z < nrows; z++)
-            for (uint32_t j = 0; j < kv; j++) y[(size_t)z * ldy + j0 + j] = 0;
-        if (si == nseg - 1)
-            for (uint32_t z = r + 1; z < d.rend && z < nrows; z++)
-                for (uint32_t j = 0; j < kv; j++) y[(size_t)z * ldy + j0 + j] = 0;
     }
-    if (nseg == 0)
-        for (uint32_t z = d.rbeg + lane; z < d.rend && z < nrows; z += WAVE)
-            for (uint32_t j = 0; j < kv; j++) y[(size_t)z * ldy + j0 + j] = 0;
+    zero_empty_rows(sk, sg, nseg, d, nrows, [&](uint32_t z) {
+        for (uint32_t j = 0; j < kv; j++) y[(size_t)z * ldy + j0 + j] = (AT)0;
+    });
 }
 
 } // namespace ispv

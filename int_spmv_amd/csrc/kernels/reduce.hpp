@@ -83,19 +83,33 @@ __device__ __forceinline__ uint32_t find_segments(const uint32_t *sk, uint16_t *
     return nseg;
 }
 
-// Writes the run result of row r and zeroes the empty rows of the block that
-// precede it (and, for the last run, follow it).
-template <typename AT>
-__device__ __forceinline__ void store_run(AT acc, uint32_t s, uint32_t nseg, const uint32_t *sk,
-                                          const uint16_t *sseg, const BlockDesc &d, AT *y,
-                                          uint32_t nrows)
+// Rows per normal block are capped at this (plan rmax) for the keyed kernels.
+constexpr uint32_t BLOCK_ROWS_MAX = 256;
+
+// Calls zero_row(r) for every row r of the block [rbeg, rend) that heads no
+// run (a row without units).  A 256-bit map in LDS marks the rows the runs
+// write; consecutive lanes then take consecutive rows, so the zeros are
+// written coalesced however long a gap of empty rows is.  Nothing happens
+// when every row of the block has a run (the common case).
+template <typename F>
+__device__ __forceinline__ void zero_empty_rows(const uint32_t *sk, const uint16_t *sseg,
+                                                uint32_t nseg, const BlockDesc &d,
+                                                uint32_t nrows, F zero_row)
 {
-    uint32_t r = sk[sseg[s]];
-    if (r < nrows) y[r] = acc;
-    uint32_t lo = (s == 0) ? d.rbeg : sk[sseg[s - 1]] + 1;
-    for (uint32_t z = lo; z < r && z < nrows; z++) y[z] = 0;
-    if (s == nseg - 1)
-        for (uint32_t z = r + 1; z < d.rend && z < nrows; z++) y[z] = 0;
+    __shared__ uint32_t rowmap[WAVES][BLOCK_ROWS_MAX / 32];
+    const uint32_t nr = d.rend - d.rbeg;
+    if (nseg >= nr) return; // wave-uniform
+    const int lane = lane_id();
+    uint32_t *map = rowmap[threadIdx.x / WAVE];
+    if (lane < (int)(BLOCK_ROWS_MAX / 32)) map[lane] = 0;
+    wave_lds_sync();
+    for (uint32_t s = lane; s < nseg; s += WAVE) {
+        const uint32_t o = sk[sseg[s]] - d.rbeg;
+        if (o < nr && o < BLOCK_ROWS_MAX) atomicOr(&map[o >> 5], 1u << (o & 31));
+    }
+    wave_lds_sync();
+    for (uint32_t z = lane; z < nr && z < BLOCK_ROWS_MAX; z += WAVE)
+        if (!((map[z >> 5] >> (z & 31)) & 1u) && d.rbeg + z < nrows) zero_row(d.rbeg + z);
 }
 
 // Keyed segmentation.  sseg needs room for nu + 1 entries.
@@ -121,13 +135,8 @@ __device__ __forceinline__ void reduce_keyed(const AT *sv, const uint32_t *sk, u
         for (uint32_t i = a; i < b; i++) acc = acc + sv[i];
         uint32_t r = sk[a];
         if (r < nrows) y[r] = acc;
-        uint32_t lo = (s == 0) ? d.rbeg : sk[sseg[s - 1]] + 1;
-        for (uint32_t z = lo; z < r && z < nrows; z++) y[z] = 0;
-        if (s == nseg - 1)
-            for (uint32_t z = r + 1; z < d.rend && z < nrows; z++) y[z] = 0;
     }
-    if (nseg == 0)
-        for (uint32_t z = d.rbeg + lane; z < d.rend && z < nrows; z += WAVE) y[z] = 0;
+    zero_empty_rows(sk, sseg, nseg, d, nrows, [&](uint32_t z) { y[z] = (AT)0; });
 }
 
 } // namespace ispv

@@ -9,7 +9,7 @@ import oracle as O
 from int_spmv_amd import device as D
 from int_spmv_amd import host as H
 
-from .util import bitwise_equal, csr_long_rows, csr_with_empty_rows, scaled
+from .util import bitwise_equal, csr_long_rows, csr_sparse_gaps, csr_with_empty_rows, scaled
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,8 @@ def _nan(eng, n, dtype=np.float64):
 def _mats():
     return [("powerlaw", H.CSR.powerlaw(n=1 << 13, target_nnz=200_000, maxlen=4096, seed=2)),
             ("empty_rows", csr_with_empty_rows()),
-            ("long_rows", csr_long_rows())]
+            ("long_rows", csr_long_rows()),
+            ("sparse_gaps", csr_sparse_gaps())]
 
 
 @pytest.fixture(scope="module")

@@ -9,7 +9,7 @@ from int_spmv_amd import device as D
 from int_spmv_amd import host as H
 from int_spmv_amd.workloads import batch_256
 
-from .util import bitwise_equal, csr_long_rows, scaled
+from .util import bitwise_equal, csr_long_rows, csr_sparse_gaps, scaled
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +39,7 @@ def test_batch_f64_equals_single(engine):
     import torch
     mats = [(m, k) for (m, k) in batch_256(seed=1, count=10, nmax=20000)]
     mats.append((csr_long_rows(), 3))  # split rows inside a batch
+    mats.append((csr_sparse_gaps(), 5))  # long runs of empty rows
     ents = _entries(engine, mats, (1, 2, 4, 8), torch.float64)
     b = D.DeviceBatch(engine, ents, D.F64, D.F64)
     b.launch()

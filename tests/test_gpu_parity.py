@@ -13,7 +13,7 @@ import oracle as O
 from int_spmv_amd import device as D
 from int_spmv_amd import host as H
 
-from .util import bitwise_equal, csr_long_rows, csr_with_empty_rows, scaled
+from .util import bitwise_equal, csr_long_rows, csr_sparse_gaps, csr_with_empty_rows, scaled
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +39,7 @@ def _matrices():
     yield "empty_rows", csr_with_empty_rows()
     yield "banded", H.CSR.banded(5000, 4, 64, seed=2)
     yield "long_rows", csr_long_rows()
+    yield "sparse_gaps", csr_sparse_gaps()
 
 
 @pytest.fixture(scope="module")

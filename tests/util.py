@@ -24,6 +24,25 @@ def csr_with_empty_rows(n=300, seed=3):
     return H.CSR.from_arrays(row, col, val)
 
 
+def csr_sparse_gaps(n=4000, seed=5):
+    """About 60% of the rows empty, plus gaps of 300 (longer than a block's
+    row cap) and 70 consecutive empty rows, and empty first and last rows:
+    the empty-row zero fill of the keyed kernels across and inside blocks."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(1, 30, size=n)
+    lens[rng.random(n) < 0.6] = 0
+    lens[1000:1300] = 0
+    lens[2000:2070] = 0
+    lens[0] = 0
+    lens[-1] = 0
+    lens[1500] = 900
+    row = np.zeros(n + 1, np.uint32)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(n, size=l, replace=False)) for l in lens]).astype(np.uint32)
+    val = rng.standard_normal(len(col))
+    return H.CSR.from_arrays(row, col, val)
+
+
 def scaled(m, x, yref, y):
     e, arg = O.scaled_err(m.row, m.col, m.val, x, yref, np.asarray(y, np.float64))
     return e
