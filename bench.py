@@ -175,7 +175,7 @@ def cpu_leg(m, x, a1, hosts, formats, y1, head_bytes, workload, quick):
     def serr(y):
         return O.scaled_err(m.row, m.col, m.val, x, yref, y)[0]
     formats["AXT_NOC_H1_F64"]["max_scaled_err_vs_csr"] = serr(y1[:n].cpu().numpy())
-    for nm in ("AXT_NOC_H1_F32", "AXT_NOC_H1_BF16"):
+    for nm in ("AXT_NOC_H1_F32", "AXT_NOC_H1_BF16", "AXT_NOC_H4_F32", "AXT_NOC_H4_BF16"):
         if nm in formats and "_y" in formats[nm]:
             yy = formats[nm].pop("_y")[:n].double().cpu().numpy()
             formats[nm]["max_scaled_err_vs_fp64"] = serr(yy)
@@ -329,17 +329,21 @@ def main():
                    {"beta": round(a.fd.beta, 4)})
             if th == 4:
                 hosts["AXT_H4"] = a
+                h4_ev = e
             del d, a, y
-        for vt, xt, nm, sv, sx in ((D.F32, D.F32, "AXT_NOC_H1_F32", 4, 4),
-                                  (D.BF16, D.F32, "AXT_NOC_H1_BF16", 2, 4)):
-            d = D.DeviceAXT(eng, a1, val_type=vt, x_type=xt)
-            y = eng.empty(n, np.float32)
-            w, e = timed(lambda: d.int_axt(y), 20, 3, torch)
-            own = M.bytes_axt(n, a1.lenAX, a1.lenSEC, sv=sv, sx=sx, sy=4)
-            record(nm, own, w, e, {"GBs_eq_fp64": round(head_bytes / e / 1e9, 1),
-                                   "vs_fp64": round(head_ev / e, 2)})
-            formats[nm]["_y"] = y
-            del d
+        a4 = hosts["AXT_H4"]
+        h4_bytes = M.bytes_axt(n, a4.lenAX, a4.lenSEC)
+        for ah, th, fb, fe in ((a1, 1, head_bytes, head_ev), (a4, 4, h4_bytes, h4_ev)):
+            for vt, xt, prec, sv, sx in ((D.F32, D.F32, "F32", 4, 4), (D.BF16, D.F32, "BF16", 2, 4)):
+                nm = f"AXT_NOC_H{th}_{prec}"
+                d = D.DeviceAXT(eng, ah, val_type=vt, x_type=xt)
+                y = eng.empty(n, np.float32)
+                w, e = timed(lambda: d.int_axt(y), 20, 3, torch)
+                own = M.bytes_axt(n, ah.lenAX, ah.lenSEC, sv=sv, sx=sx, sy=4)
+                record(nm, own, w, e, {"GBs_eq_fp64": round(fb / e / 1e9, 1),
+                                       "vs_fp64": round(fe / e, 2)})
+                formats[nm]["_y"] = y
+                del d
         # column-map form: 1 and 8 right-hand sides, adjoint on the transposed map
         ag = H.AXT(m, x, th=1, want_acol=True)
         g = D.DeviceAXTGather(eng, ag)

@@ -467,6 +467,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
     const XT *xb = PLANAR ? axv : (const XT *)av;
     LaneSum<AT> lacc; // long rows
+    if (d.k == 0) stage_keys(sec, d, nu, s_key[w]);
     for (uint32_t Tb = T0; Tb < T1; Tb += 16) {
         const uint32_t Tr = Tb + lane / 4, u0 = 8 * Tr + C;
         const bool ok0 = u0 >= d.ubeg && u0 < d.uend;
@@ -481,15 +482,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
             if (ok1) lacc.add(a1);
             continue;
         }
-        const u2v kk = ntload(reinterpret_cast<const u2v *>(sec + 8 * (size_t)T + C));
-        if (ok0) {
-            s_val[w][u0 - d.ubeg] = a0;
-            s_key[w][u0 - d.ubeg] = kk[0];
-        }
-        if (ok1) {
-            s_val[w][u0 + 1 - d.ubeg] = a1;
-            s_key[w][u0 + 1 - d.ubeg] = kk[1];
-        }
+        if (ok0) s_val[w][u0 - d.ubeg] = a0;
+        if (ok1) s_val[w][u0 + 1 - d.ubeg] = a1;
     }
     if (d.k != 0) {
         finish_long<AT>(wave_sum<AT>(lacc.value()), d, bid, y, nrows, partial);
@@ -602,43 +596,57 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
     const uint32_t nu = d.uend - d.ubeg, p = lane & 7, C = 2 * (p & 3);
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
     double lacc = 0.0; // long rows
-    for (uint32_t Tb = T0; Tb < T1; Tb += 8) {
-        const uint32_t Tr = Tb + lane / 8, u0 = 8 * Tr + C;
-        const bool ok0 = p < 4 && Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
-        const bool ok1 = p < 4 && Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
-        const size_t T = Tr < T1 ? Tr : T1 - 1;
-        double a0 = 0.0, a1 = 0.0;
+    if (d.k == 0) stage_keys(sec, d, nu, s_key[w]); // unit u = column 8T+C has key sec[u]
+    // G groups of 8 tiles per step: a normal block is 16 tiles, so two
+    // groups put th = 2 and 4 blocks in flight at once
+    constexpr int G = (TH == 0 || TH >= 8) ? 1 : 2;
+    for (uint32_t Tb = T0; Tb < T1; Tb += 8 * G) {
+        double a0[G], a1[G];
         if constexpr (TH > 0) {
-            d2v v[TH];
+            d2v v[G][TH];
 #pragma unroll
-            for (int R = 0; R < TH; R++) v[R] = ntload(reinterpret_cast<const d2v *>(ax + (T * TH + R) * 16 + 2 * p));
+            for (int g = 0; g < G; g++) {
+                const uint32_t Tr = Tb + 8 * g + lane / 8;
+                const size_t T = Tr < T1 ? Tr : T1 - 1;
 #pragma unroll
-            for (int R = 0; R < TH; R++) {
-                double x0 = dpp_f64<DPP_ROW_SHL4>(v[R][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[R][1]);
-                a0 = fma(v[R][0], x0, a0);
-                a1 = fma(v[R][1], x1, a1);
+                for (int R = 0; R < TH; R++)
+                    v[g][R] = ntload(reinterpret_cast<const d2v *>(ax + (T * TH + R) * 16 + 2 * p));
+            }
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                a0[g] = 0.0;
+                a1[g] = 0.0;
+#pragma unroll
+                for (int R = 0; R < TH; R++) {
+                    double x0 = dpp_f64<DPP_ROW_SHL4>(v[g][R][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[g][R][1]);
+                    a0[g] = fma(v[g][R][0], x0, a0[g]);
+                    a1[g] = fma(v[g][R][1], x1, a1[g]);
+                }
             }
         } else {
+            const uint32_t Tr = Tb + lane / 8;
+            const size_t T = Tr < T1 ? Tr : T1 - 1;
+            a0[0] = 0.0;
+            a1[0] = 0.0;
             for (uint32_t R = 0; R < th; R++) {
                 d2v v = ntload(reinterpret_cast<const d2v *>(ax + (T * th + R) * 16 + 2 * p));
                 double x0 = dpp_f64<DPP_ROW_SHL4>(v[0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[1]);
-                a0 = fma(v[0], x0, a0);
-                a1 = fma(v[1], x1, a1);
+                a0[0] = fma(v[0], x0, a0[0]);
+                a1[0] = fma(v[1], x1, a1[0]);
             }
         }
-        if (d.k != 0) {
-            if (ok0) lacc = lacc + a0;
-            if (ok1) lacc = lacc + a1;
-            continue;
-        }
-        const u2v kk = ntload(reinterpret_cast<const u2v *>(sec + 8 * T + C));
-        if (ok0) {
-            s_val[w][u0 - d.ubeg] = a0;
-            s_key[w][u0 - d.ubeg] = kk[0];
-        }
-        if (ok1) {
-            s_val[w][u0 + 1 - d.ubeg] = a1;
-            s_key[w][u0 + 1 - d.ubeg] = kk[1];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const uint32_t Tr = Tb + 8 * g + lane / 8, u0 = 8 * Tr + C;
+            const bool ok0 = p < 4 && Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
+            const bool ok1 = p < 4 && Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
+            if (d.k != 0) {
+                if (ok0) lacc = lacc + a0[g];
+                if (ok1) lacc = lacc + a1[g];
+            } else {
+                if (ok0) s_val[w][u0 - d.ubeg] = a0[g];
+                if (ok1) s_val[w][u0 + 1 - d.ubeg] = a1[g];
+            }
         }
     }
     if (d.k != 0) {
@@ -647,6 +655,174 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
     }
     wave_lds_sync();
     reduce_keyed<double>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
+}
+
+// th > 1 (even), FP32 reference layout, full-line mapping: a 128-byte line
+// holds brick-rows R, R+1 of one tile (64 bytes each).  Lane (tile Tb + l/8,
+// part p = l%8) reads 16 bytes: values (p%4 < 2) or x (p%4 >= 2) of columns
+// 4(p%4 % 2).. of brick-row R + p/4.  Lanes p = 0, 1 run the fma chains of
+// their four columns over the brick-rows in order, taking the x of row R and
+// the values and x of row R+1 from the other lanes of the line by DPP.
+template <int TH>
+__global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl32_kernel(
+    const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
+    const float *__restrict__ ax, const uint32_t *__restrict__ sec, float *__restrict__ y,
+    uint32_t nrows, double *partial)
+{
+    static_assert(TH >= 2 && TH % 2 == 0, "brick-row pairs");
+    __shared__ float s_val[WAVES][AXT_UMAX];
+    __shared__ uint32_t s_key[WAVES][AXT_UMAX];
+    __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const uint32_t bid = block_index(nlong_wg);
+    if (bid >= nblocks) return;
+    const BlockDesc d = blocks[bid];
+    const uint32_t nu = d.uend - d.ubeg, p = lane & 7;
+    const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
+    LaneSum<float> lacc; // long rows
+    if (d.k == 0) stage_keys(sec, d, nu, s_key[w]);
+    constexpr int L = TH / 2;           // lines per tile
+    constexpr int G = L >= 8 ? 1 : 2;   // tile groups of 8 tiles per step (block = 16 tiles)
+    for (uint32_t Tb = T0; Tb < T1; Tb += 8 * G) {
+        float4 v[G][L];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const uint32_t Tr = Tb + 8 * g + lane / 8;
+            const size_t T = Tr < T1 ? Tr : T1 - 1;
+#pragma unroll
+            for (int i = 0; i < L; i++)
+                v[g][i] = f4load(ax + (T * TH + 2 * i + p / 4) * 16 + 4 * (p % 4));
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int i = 0; i < L; i++) {
+                const float4 x0 = f4dpp<DPP_ROW_SHL2>(v[g][i]); // x of row 2i
+                const float4 v1 = f4dpp<DPP_ROW_SHL4>(v[g][i]); // values of row 2i+1
+                const float4 x1 = f4dpp<DPP_ROW_SHL4>(x0);      // x of row 2i+1
+                a.x = fmaf(v[g][i].x, x0.x, a.x);
+                a.y = fmaf(v[g][i].y, x0.y, a.y);
+                a.z = fmaf(v[g][i].z, x0.z, a.z);
+                a.w = fmaf(v[g][i].w, x0.w, a.w);
+                a.x = fmaf(v1.x, x1.x, a.x);
+                a.y = fmaf(v1.y, x1.y, a.y);
+                a.z = fmaf(v1.z, x1.z, a.z);
+                a.w = fmaf(v1.w, x1.w, a.w);
+            }
+            const uint32_t Tr = Tb + 8 * g + lane / 8, u0 = 8 * Tr + 4 * p;
+            if (p < 2 && Tr < T1) {
+                const float c[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t u = u0 + q;
+                    if (u < d.ubeg || u >= d.uend) continue;
+                    if (d.k != 0) lacc.add(c[q]);
+                    else s_val[w][u - d.ubeg] = c[q];
+                }
+            }
+        }
+    }
+    if (d.k != 0) {
+        finish_long<float>(wave_sum_f32(lacc.value()), d, bid, y, nrows, partial);
+        return;
+    }
+    wave_lds_sync();
+    reduce_keyed<float>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
+}
+
+#ifndef ISPV_PLANAR_STEPS
+#define ISPV_PLANAR_STEPS 4
+#endif
+// Four stored values (bf16 or FP32) of the value plane as floats.
+template <typename VT>
+__device__ __forceinline__ float4 load4_values(const VT *p)
+{
+    if constexpr (sizeof(VT) == 2) {
+        const u2v b = ntload(reinterpret_cast<const u2v *>(p));
+        return make_float4(__uint_as_float(b[0] << 16), __uint_as_float(b[0] & 0xFFFF0000u),
+                           __uint_as_float(b[1] << 16), __uint_as_float(b[1] & 0xFFFF0000u));
+    } else {
+        return f4load(reinterpret_cast<const float *>(p));
+    }
+}
+
+// Planar storage (value plane of bf16 or FP32, x plane of FP32), tile height
+// TH in {1, 2, 4, 8, 16}.  Lane l of load step j takes half-brick-row
+// h = H0 + 64 j + l, i.e. the four slots 4h..4h+3 (columns 4(h%2).. of
+// brick-row h/2), so every load instruction reads contiguous memory: 64 x 4
+// values and 64 x 16 bytes of x.  TH = 1: the two halves of a tile meet by a
+// quad exchange.  TH > 1: the 2 TH lanes of a tile are aligned in a row of
+// 16 (H0 is a multiple of 2 TH) and row shifts sum each column over the
+// brick-rows into the tile's first two lanes (TH = 16 adds the second row
+// with one shuffle).  FP32 accumulation; long rows Kahan-Babuska-Neumaier.
+template <typename VT, int TH>
+__global__ __launch_bounds__(BLOCK_THREADS) void axt_planar_kernel(
+    const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
+    const VT *__restrict__ av, const float *__restrict__ xp, const uint32_t *__restrict__ sec,
+    float *__restrict__ y, uint32_t nrows, double *partial)
+{
+    static_assert(TH == 1 || TH == 2 || TH == 4 || TH == 8 || TH == 16, "tile height");
+    __shared__ float s_val[WAVES][AXT_UMAX];
+    __shared__ uint32_t s_key[WAVES][AXT_UMAX];
+    __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const uint32_t bid = block_index(nlong_wg);
+    if (bid >= nblocks) return;
+    const BlockDesc d = blocks[bid];
+    const uint32_t nu = d.uend - d.ubeg;
+    const size_t T0 = TH == 1 ? d.ubeg : d.ubeg / 8, T1 = TH == 1 ? d.uend : (d.uend + 7) / 8;
+    const size_t H0 = T0 * 2 * TH, H1 = T1 * 2 * TH;
+    LaneSum<float> lacc; // long rows
+    if (d.k == 0) stage_keys(sec, d, nu, s_key[w]);
+    constexpr int U = ISPV_PLANAR_STEPS;
+    for (size_t hb = H0; hb < H1; hb += 64 * U) {
+        float4 v[U], x[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const size_t h = hb + 64 * j + lane, hc = h < H1 ? h : H1 - 1;
+            v[j] = load4_values<VT>(av + hc * 4);
+            x[j] = f4load(xp + hc * 4);
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const size_t h = hb + 64 * j + lane;
+            float c[4] = {v[j].x * x[j].x, v[j].y * x[j].y, v[j].z * x[j].z, v[j].w * x[j].w};
+            if constexpr (TH == 1) {
+                float t = (c[0] + c[1]) + (c[2] + c[3]);
+                t = t + dpp_f32<DPP_QUAD_XOR1>(t);
+                if (h < H1 && (lane & 1) == 0) {
+                    const uint32_t u = (uint32_t)(h / 2);
+                    if (d.k != 0) lacc.add(t);
+                    else s_val[w][u - d.ubeg] = t;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    c[q] = c[q] + dpp_f32<DPP_ROW_SHL2>(c[q]);
+                    if constexpr (TH >= 4) c[q] = c[q] + dpp_f32<DPP_ROW_SHL4>(c[q]);
+                    if constexpr (TH >= 8) c[q] = c[q] + dpp_f32<DPP_ROW_SHL8>(c[q]);
+                    if constexpr (TH >= 16) c[q] = c[q] + __shfl_down(c[q], 16, WAVE);
+                }
+                if (h < H1 && (lane & (2 * TH - 2)) == 0) { // brick-row 0 of the tile
+                    const uint32_t u0 = (uint32_t)(8 * (h / (2 * TH)) + 4 * (h & 1));
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t u = u0 + q;
+                        if (u < d.ubeg || u >= d.uend) continue;
+                        if (d.k != 0) lacc.add(c[q]);
+                        else s_val[w][u - d.ubeg] = c[q];
+                    }
+                }
+            }
+        }
+    }
+    if (d.k != 0) {
+        finish_long<float>(wave_sum_f32(lacc.value()), d, bid, y, nrows, partial);
+        return;
+    }
+    wave_lds_sync();
+    reduce_keyed<float>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
 }
 
 template <typename VT, typename XT, typename AT, bool PLANAR>
@@ -668,9 +844,32 @@ static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, con
         } else {
             return INTSPMV_ERR_UNSUPPORTED;
         }
+    } else if (PLANAR && sizeof(VT) <= 4 && sizeof(XT) == 4 && sizeof(AT) == 4 &&
+               (th == 1 || th == 2 || th == 4 || th == 8 || th == 16)) {
+        if constexpr (PLANAR && sizeof(VT) <= 4 && sizeof(XT) == 4 && sizeof(AT) == 4) {
+            auto xf = (const float *)xp;
+            auto yf = (float *)y;
+            switch (th) {
+            case 1: axt_planar_kernel<VT, 1><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, xf, sec, yf, p->nrows, p->d_partial); break;
+            case 2: axt_planar_kernel<VT, 2><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, xf, sec, yf, p->nrows, p->d_partial); break;
+            case 4: axt_planar_kernel<VT, 4><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, xf, sec, yf, p->nrows, p->d_partial); break;
+            case 8: axt_planar_kernel<VT, 8><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, xf, sec, yf, p->nrows, p->d_partial); break;
+            default: axt_planar_kernel<VT, 16><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, xf, sec, yf, p->nrows, p->d_partial); break;
+            }
+        }
     } else if (th == 1) {
         axt_h1_kernel<VT, XT, AT, PLANAR><<<grid, blk, 0, s>>>(
             p->d_blocks, nl, nb, a, x, sec, yy, p->nrows, p->d_partial);
+    } else if constexpr (!PLANAR && sizeof(VT) == 4 && sizeof(XT) == 4) {
+        switch (th) {
+        case 2: axt_noc_fl32_kernel<2><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial); break;
+        case 4: axt_noc_fl32_kernel<4><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial); break;
+        case 8: axt_noc_fl32_kernel<8><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial); break;
+        case 16: axt_noc_fl32_kernel<16><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial); break;
+        default:
+            axt_noc_kernel<VT, XT, AT, PLANAR, 0><<<grid, blk, 0, s>>>(
+                p->d_blocks, nl, nb, a, x, sec, yy, p->nrows, p->d_partial, th);
+        }
     } else if constexpr (!PLANAR && sizeof(VT) == 8) {
         switch (th) {
         case 2: axt_noc_fl_kernel<2><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;

@@ -168,7 +168,7 @@ def test_axt_tiny10_exact(engine, tiny10):
         assert np.array_equal(_host(y, m.nrows), np.array(fx["y_exact"], np.float64))
 
 
-@pytest.mark.parametrize("th", [1, 4])
+@pytest.mark.parametrize("th", [1, 2, 3, 4, 8, 16])
 @pytest.mark.parametrize("vt,xt,tol", [(D.F32, D.F32, TOL32), (D.BF16, D.F32, TOL16)])
 def test_axt_precision(engine, mats, th, vt, xt, tol):
     for name, m, x, yr in mats:
