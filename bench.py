@@ -306,18 +306,22 @@ def main():
     record("AXT_NOC_H1_F64", head_bytes, wall, ev,
            {"beta": round(a1.fd.beta, 4), "plan": d1.plan.info()})
     head_wall, head_ev = wall, ev
-    # per-launch events: median warm, and cold (a 1 GiB buffer written between launches
-    # evicts the matrix from the 256 MB MALL and the L2s)
-    warm = timed_each(lambda: d1.int_axt(y1), args.steps, 1, torch)
-    scratch = torch.empty(1 << 27, dtype=torch.float64, device=eng.torch_device)
-    cold = timed_each(lambda: d1.int_axt(y1), args.steps, 1, torch, between=lambda: scratch.fill_(1.0))
-    del scratch
-    med, cmed = warm[len(warm) // 2], cold[len(cold) // 2]
-    formats["AXT_NOC_H1_F64"].update({
-        "us_median": round(med * 1e6, 2), "GBs_median": round(head_bytes / med / 1e9, 1),
-        "us_cold_median": round(cmed * 1e6, 2), "GBs_cold": round(head_bytes / cmed / 1e9, 1)})
-    log(f"[bench] AXT_NOC_H1_F64 per-launch median {med * 1e6:.2f} us, cold {cmed * 1e6:.2f} us")
     hosts = {}
+    if args.quick:  # headline launches only (profiling)
+        warm = cold = None
+    else:
+        # per-launch events: median warm, and cold (a 1 GiB buffer written between
+        # launches evicts the matrix from the 256 MB MALL and the L2s)
+        warm = timed_each(lambda: d1.int_axt(y1), args.steps, 1, torch)
+        scratch = torch.empty(1 << 27, dtype=torch.float64, device=eng.torch_device)
+        cold = timed_each(lambda: d1.int_axt(y1), args.steps, 1, torch,
+                          between=lambda: scratch.fill_(1.0))
+        del scratch
+        med, cmed = warm[len(warm) // 2], cold[len(cold) // 2]
+        formats["AXT_NOC_H1_F64"].update({
+            "us_median": round(med * 1e6, 2), "GBs_median": round(head_bytes / med / 1e9, 1),
+            "us_cold_median": round(cmed * 1e6, 2), "GBs_cold": round(head_bytes / cmed / 1e9, 1)})
+        log(f"[bench] AXT_NOC_H1_F64 per-launch median {med * 1e6:.2f} us, cold {cmed * 1e6:.2f} us")
 
     if not args.quick:
         for th in (2, 4, 8, 16):

@@ -25,6 +25,9 @@ namespace ispv {
 #ifndef ISPV_H1_STEPS
 #define ISPV_H1_STEPS 4
 #endif
+#ifndef ISPV_H1_F64_STEPS
+#define ISPV_H1_F64_STEPS 8 // FP64 normal blocks: the whole 128-tile block in flight
+#endif
 constexpr uint32_t AXT_UMAX = ISPV_AXT_UMAX; // units (tiles or columns) per block
 constexpr uint32_t AXT_RMAX = 256;  // rows per block
 static_assert(AXT_RMAX <= BLOCK_ROWS_MAX, "keyed blocks map their rows in a 256-bit LDS map");
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
         // 2p,2p+1 (p<4) or x 2(p-4),2(p-4)+1 (p>=4); one DPP row shift brings
         // the x pair to the value lanes, whose quads then form the tree.
         const uint32_t p = lane & 7;
-        constexpr int U = 2 * ISPV_H1_STEPS;
+        constexpr int U = 2 * ISPV_H1_F64_STEPS;
         stage_keys(sec, d, nu, s_key[w]);
         for (uint32_t base = 0; base < nu; base += 8 * U) {
             d2v v[U];
@@ -732,7 +735,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl32_kernel(
 }
 
 #ifndef ISPV_PLANAR_STEPS
-#define ISPV_PLANAR_STEPS 4
+#define ISPV_PLANAR_STEPS 4 // tile height 1
+#endif
+#ifndef ISPV_PLANAR_STEPS_TH
+#define ISPV_PLANAR_STEPS_TH 2 // tile height > 1
 #endif
 // Four stored values (bf16 or FP32) of the value plane as floats.
 template <typename VT>
@@ -775,7 +781,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_planar_kernel(
     const size_t H0 = T0 * 2 * TH, H1 = T1 * 2 * TH;
     LaneSum<float> lacc; // long rows
     if (d.k == 0) stage_keys(sec, d, nu, s_key[w]);
-    constexpr int U = ISPV_PLANAR_STEPS;
+    constexpr int U = TH == 1 ? ISPV_PLANAR_STEPS : ISPV_PLANAR_STEPS_TH;
     for (size_t hb = H0; hb < H1; hb += 64 * U) {
         float4 v[U], x[U];
 #pragma unroll
