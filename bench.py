@@ -38,6 +38,10 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--batch-count", type=int, default=256,
                    help="matrices in the batched-launch measurement (0 = skip)")
+    p.add_argument("--batch-tune", choices=("device", "model"), default="model",
+                   help="batch autotuner: byte model only (default; 128 units per wave), or "
+                        "byte model + device timing of each matrix alone (tile height, RHS "
+                        "blocking and units per wave)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run a plain device copy of this many MiB (counter calibration)")
     return p.parse_args()
@@ -85,11 +89,16 @@ def timed_each(fn, steps, warmup, torch, between=None):
     return sorted(a.elapsed_time(b) * 1e-3 for a, b in evs)
 
 
-def bench_batch(eng, count, torch, log):
+def bench_batch(eng, count, torch, log, tune="model"):
     """256 matrices (n log-uniform in [1e3, 2e5], 4..64 entries per row, 70 %
-    banded), FP32 values and x, k in {1,4,8,16,32} each; per-matrix tile height
-    and right-hand-side blocking from the autotuner's byte model; one
-    persistent launch captured in a hipGraph and replayed."""
+    banded), FP32 values and x, k in {1,4,8,16,32} each; per-matrix tile
+    height, right-hand-side blocking and units per wave from the autotuner
+    (tune="device": byte model + device timing; "model": byte model only, 128
+    units per wave); one persistent launch captured in a hipGraph and
+    replayed."""
+    import collections
+
+    from int_spmv_amd.tune import tune_axt
     import numpy as np
 
     from int_spmv_amd import device as D
@@ -98,13 +107,20 @@ def bench_batch(eng, count, torch, log):
     from int_spmv_amd.workloads import batch_256
     t0 = time.perf_counter()
     mats = batch_256(seed=0, count=count)
+    gen_s = time.perf_counter() - t0
     ents, flops, bytes_, rows, nnz = [], 0.0, 0, 0, 0
+    chosen = collections.Counter()
+    t1 = time.perf_counter()
     for i, (m, k) in enumerate(mats):
-        c = H.tune_model(m, k, 4, 4)[0]
+        if tune == "device":
+            c = tune_axt(eng, m, k, D.F32, D.F32, top=2, reps=3)
+        else:
+            c = dict(H.tune_model(m, k, 4, 4)[0], umax=128)
+        chosen[(c["tileH"], c["kc"], c["umax"])] += 1
         a = H.AXT(m, np.zeros(m.nrows), th=c["tileH"], want_acol=True)
         X = torch.rand(m.nrows * k, dtype=torch.float32, device=eng.torch_device)
         Y = torch.empty(m.nrows * k, dtype=torch.float32, device=eng.torch_device)
-        ents.append(dict(a=a, X=X, Y=Y, k=k, kc=c["kc"]))
+        ents.append(dict(a=a, X=X, Y=Y, k=k, kc=c["kc"], umax=c["umax"]))
         flops += M.flops(m.nnz, k)
         bytes_ += M.bytes_axt_gather(m.nrows, a.lenAX, a.lenSEC, k=k, sv=4, sx=4, sy=4)
         rows += m.nrows
@@ -113,7 +129,7 @@ def bench_batch(eng, count, torch, log):
     for e in ents:
         e.pop("a")
     info = b.info()
-    setup = time.perf_counter() - t0
+    tune_s = time.perf_counter() - t1
     b.capture()
     for _ in range(3):
         b.replay()
@@ -129,7 +145,9 @@ def bench_batch(eng, count, torch, log):
     r = {"matrices": count, "rows": rows, "nnz": nnz, "us": round(t * 1e6, 1),
          "GFLOPs": round(flops / t / 1e9, 1), "GBs": round(bytes_ / t / 1e9, 1),
          "pct_hbm": round(100 * bytes_ / t / 1e9 / M.HBM_PEAK_GBS, 1), "items": info["nitems"],
-         "grid": info["grid"], "setup_s": round(setup, 1)}
+         "grid": info["grid"], "generate_s": round(gen_s, 1), "tune_build_s": round(tune_s, 1),
+         "tuner": tune,
+         "choices": {f"th{t}_kc{kc}_u{u}": c for (t, kc, u), c in sorted(chosen.items())}}
     log(f"[bench] BATCH: {r}")
     return r
 
@@ -404,7 +422,7 @@ def main():
 
     batch = None
     if not args.quick and args.batch_count > 0:
-        batch = bench_batch(eng, args.batch_count, torch, log)
+        batch = bench_batch(eng, args.batch_count, torch, log, args.batch_tune)
 
     # ---- CPU baseline + correctness (oracle as the checker) -------------------
     cpu = None

@@ -444,12 +444,19 @@ class BatchMatrix(C.Structure):
                 ("umax", _u32), ("con_host", _vp)]
 
 
+def upload_batch_arrays(eng: Engine, a, val_type: int):
+    """Device copies of a column-map AXT's value plane, acol and sec."""
+    return eng.upload(_value_plane(a, val_type)), eng.upload(a.acol), eng.upload(a.sec)
+
+
 class DeviceBatch:
     """Many column-map AXT matrices with their own tile heights and 1..32
     right-hand sides in one persistent launch (optionally a hipGraph).
 
     entries: list of dicts with keys a (host AXT built with want_acol), X and
-    Y (device tensors, n*k row-major), k and optionally kc."""
+    Y (device tensors, n*k row-major), k and optionally kc, umax (units per
+    wave block) and dev (the (aval, acol, sec) device arrays of a, from
+    upload_batch_arrays, so that entries can share them)."""
 
     def __init__(self, eng: Engine, entries, val_type: int = F32, x_type: int = F32):
         self.eng = eng
@@ -457,8 +464,7 @@ class DeviceBatch:
         arr = (BatchMatrix * len(entries))()
         for i, e in enumerate(entries):
             a = e["a"]
-            aval = eng.upload(_value_plane(a, val_type))
-            acol, sec = eng.upload(a.acol), eng.upload(a.sec)
+            aval, acol, sec = e.get("dev") or upload_batch_arrays(eng, a, val_type)
             con = np.ascontiguousarray(a.con)
             self._keep += [aval, acol, sec, con, e["X"], e["Y"]]
             k = e["k"]

@@ -85,5 +85,30 @@ def test_autotuner_picks_a_measured_candidate(engine):
     m = H.CSR.banded(30000, 4, 64, band_frac=0.7, band_w=1500, seed=3)
     best = tune_axt(engine, m, 8, D.F32, D.F32, top=3, reps=3)
     assert best["tileH"] in (1, 2, 4, 8, 16) and best["kc"] in (1, 2, 4, 8)
+    assert best["umax"] in (32, 64, 128)
     assert best["us"] > 0 and len(best["ranked"]) == 3
     assert (best["tileH"], best["kc"]) in [(c["tileH"], c["kc"]) for c in best["ranked"]]
+
+
+def test_batch_wave_mapping_is_exact(engine):
+    """Every block size the tuner may pick gives the single-matrix bits."""
+    import torch
+    m = H.CSR.powerlaw(n=6000, target_nnz=90_000, maxlen=1500, seed=4)
+    k = 3
+    X = np.random.default_rng(4).uniform(-1, 1, (m.nrows, k))
+    Xd = torch.tensor(X.reshape(-1), dtype=torch.float64, device=engine.torch_device)
+    a = H.AXT(m, np.zeros(m.nrows), th=1, want_acol=True)
+    g = D.DeviceAXTGather(engine, a, D.F64, D.F64)
+    Yref = torch.full((m.nrows * k,), float("nan"), dtype=torch.float64, device=engine.torch_device)
+    g.matvec(Xd, Yref, k=k, kc=4)
+    for umax in (32, 64, 128):
+        Y = torch.full_like(Yref, float("nan"))
+        b = D.DeviceBatch(engine, [dict(a=a, X=Xd, Y=Y, k=k, kc=4, umax=umax)], D.F64, D.F64)
+        b.launch()
+        torch.cuda.synchronize()
+        yb, yr = Y.cpu().numpy(), Yref.cpu().numpy()
+        yref = np.stack([O.cf_csr(m.row, m.col, m.val, X[:, j].copy()) for j in range(k)], 1)
+        for j in range(k):
+            assert scaled(m, X[:, j].copy(), yref[:, j], yb.reshape(-1, k)[:, j]) <= 1e-12, umax
+        if umax == 128:
+            assert bitwise_equal(yb, yr)
