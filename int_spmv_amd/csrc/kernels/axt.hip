@@ -25,8 +25,21 @@ namespace ispv {
 #ifndef ISPV_H1_STEPS
 #define ISPV_H1_STEPS 4
 #endif
+// FP64 reference layout, th = 1: blocks of 64 tiles (8 KB), all in flight at
+// once, and rows above 48 tiles on waves of their own (measured against 128 /
+// 64: 85.3 -> 83.8 us on the power-law matrix, 173.7 -> 168.0 us on the
+// stencil).
+#ifndef ISPV_H1_F64_UMAX
+#define ISPV_H1_F64_UMAX 64
+#endif
+#ifndef ISPV_H1_F64_LONG
+#define ISPV_H1_F64_LONG 48
+#endif
+#ifndef ISPV_H1_RED_UMAX
+#define ISPV_H1_RED_UMAX 128 // th = 1, FP32 / bf16 storage and the column-map form
+#endif
 #ifndef ISPV_H1_F64_STEPS
-#define ISPV_H1_F64_STEPS 8 // FP64 normal blocks: the whole 128-tile block in flight
+#define ISPV_H1_F64_STEPS (ISPV_H1_F64_UMAX / 16)
 #endif
 constexpr uint32_t AXT_UMAX = ISPV_AXT_UMAX; // units (tiles or columns) per block
 constexpr uint32_t AXT_RMAX = 256;  // rows per block
@@ -258,22 +271,24 @@ __device__ __forceinline__ AT h1_long(const VT *av, const XT *xb, const BlockDes
     return wave_sum<AT>(acc.value());
 }
 
-// Stages a normal block's sec words (th = 1: one per tile) in LDS with one
-// lane per tile, so the keys cost two coalesced loads per block and no
-// registers across the value loads.
+// Stages a normal block's sec words (one per unit, nu <= UMAX) in LDS with
+// one lane per unit, so the keys cost one or two coalesced loads per block
+// and no registers across the value loads.
+template <uint32_t UMAX = AXT_UMAX>
 __device__ __forceinline__ void stage_keys(const uint32_t *sec, const BlockDesc &d, uint32_t nu,
                                            uint32_t *sk)
 {
+    static_assert(UMAX % WAVE == 0 && UMAX <= AXT_UMAX, "key staging");
     if (nu == 0) return;
     const int lane = lane_id();
-    uint32_t kk[AXT_UMAX / WAVE];
+    uint32_t kk[UMAX / WAVE];
 #pragma unroll
-    for (int j = 0; j < (int)(AXT_UMAX / WAVE); j++) {
+    for (int j = 0; j < (int)(UMAX / WAVE); j++) {
         const uint32_t t = j * WAVE + lane;
         kk[j] = ntload(sec + d.ubeg + (t < nu ? t : nu - 1));
     }
 #pragma unroll
-    for (int j = 0; j < (int)(AXT_UMAX / WAVE); j++)
+    for (int j = 0; j < (int)(UMAX / WAVE); j++)
         if (j * WAVE + lane < nu) sk[j * WAVE + lane] = kk[j];
 }
 
@@ -308,7 +323,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
         // the x pair to the value lanes, whose quads then form the tree.
         const uint32_t p = lane & 7;
         constexpr int U = 2 * ISPV_H1_F64_STEPS;
-        stage_keys(sec, d, nu, s_key[w]);
+        stage_keys<ISPV_H1_F64_UMAX>(sec, d, nu, s_key[w]);
         for (uint32_t base = 0; base < nu; base += 8 * U) {
             d2v v[U];
 #pragma unroll
@@ -966,8 +981,13 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
         free(uptr);
         return INTSPMV_ERR_ARG;
     }
-    uint32_t long_units = th == 1 ? ISPV_H1_LONG : (512 / th < 4 ? 4 : (512 / th > 64 ? 64 : 512 / th));
-    int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, AXT_UMAX, AXT_RMAX, long_units,
+    // block size and long-row threshold per kernel (see the kernels' notes)
+    const bool h1_f64 = th == 1 && D.layout == INTSPMV_AXT_INTERLEAVED &&
+                        D.val_type == INTSPMV_F64 && D.x_type == INTSPMV_F64;
+    const uint32_t umax = h1_f64 ? ISPV_H1_F64_UMAX : th == 1 ? ISPV_H1_RED_UMAX : AXT_UMAX;
+    uint32_t long_units = th == 1 ? (h1_f64 ? ISPV_H1_F64_LONG : ISPV_H1_LONG)
+                                  : (512 / th < 4 ? 4 : (512 / th > umax / 2 ? umax / 2 : 512 / th));
+    int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, umax, AXT_RMAX, long_units,
                              LONG_SLOTS / ths * (th == 1 ? 1 : 8), plan);
     free(uptr);
     if (st) return st;

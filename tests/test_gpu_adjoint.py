@@ -54,7 +54,7 @@ def test_gather_k1_equals_embedded(engine, mats, th):
         g1, g2 = _host(y1, m.nrows), _host(y2, m.nrows)
         per = 8 if th == 1 else th
         units = (np.diff(m.row.astype(np.int64)) + per - 1) // per
-        ok = units <= (64 if th == 1 else 64)
+        ok = units <= (48 if th == 1 else 64)  # rows in normal blocks of both plans
         assert bitwise_equal(g1[ok], g2[ok]), (name, th)
         assert scaled(m, x, O.cf_csr(m.row, m.col, m.val, x), g1) <= 1e-12, (name, th)
 
