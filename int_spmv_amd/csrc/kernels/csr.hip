@@ -37,11 +37,17 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_ordered_kernel(
     y[r] = acc;
 }
 
+// PAIRS (nnz >= 2): phase 1 reads aligned pairs of entries (16 bytes of values
+// and 8 of columns per lane, half the load instructions of one entry per lane).
+// A pair never reaches past the arrays: pair starts are clamped to the last
+// whole pair, and with odd nnz the last entry is fetched on its own (one
+// uniform load) and substituted.
+template <bool PAIRS>
 __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
     const double *__restrict__ val, const uint32_t *__restrict__ col,
     const uint32_t *__restrict__ row, const double *__restrict__ x, double *__restrict__ y,
-    uint32_t nrows, double *partial)
+    uint32_t nrows, double *partial, uint32_t nnz)
 {
     __shared__ double s_v[WAVES][CSR_UMAX];
     __shared__ double s_x[WAVES][CSR_UMAX];
@@ -78,25 +84,72 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
         ra = row[r0];
         rb = row[r0 + 1];
     }
-    for (uint32_t base = 0; base < nu; base += WAVE * U) {
-        double v[U];
-        uint32_t c[U];
-#pragma unroll
-        for (int j = 0; j < U; j++) {
-            uint32_t i = base + j * WAVE + lane;
-            i = i < nu ? i : nu - 1; // branch-free: re-read the last entry
-            v[j] = ntload(val + d.ubeg + i);
-            c[j] = ntload(col + d.ubeg + i);
+    if constexpr (PAIRS) {
+        constexpr int U2 = U / 2;
+        const uint32_t pb = d.ubeg & ~1u;             // first pair start (even)
+        const uint32_t np = (d.uend - pb + 1) / 2;    // pairs touching the block
+        const uint32_t gmax = (nnz - 2) & ~1u;        // last pair inside the arrays
+        double tv = 0.0;
+        uint32_t tc = 0;
+        if ((nnz & 1) && d.uend == nnz) { // uniform: this block holds the lone last entry
+            tv = val[nnz - 1];
+            tc = col[nnz - 1];
         }
-        double xv[U];
+        for (uint32_t base = 0; base < np; base += WAVE * U2) {
+            d2v v[U2];
+            u2v c[U2];
 #pragma unroll
-        for (int j = 0; j < U; j++) xv[j] = x[c[j]];
+            for (int j = 0; j < U2; j++) {
+                uint32_t q = base + j * WAVE + lane;
+                q = q < np ? q : np - 1; // branch-free: re-read the last pair
+                const uint32_t g = pb + 2 * q, gl = g < gmax ? g : gmax;
+                v[j] = ntload(reinterpret_cast<const d2v *>(val + gl));
+                c[j] = ntload(reinterpret_cast<const u2v *>(col + gl));
+                if (g != gl) { // only the odd last entry: its pair is not read
+                    v[j][0] = tv;
+                    c[j][0] = tc;
+                }
+            }
+            double x0[U2], x1[U2];
 #pragma unroll
-        for (int j = 0; j < U; j++) {
-            uint32_t i = base + j * WAVE + lane;
-            if (i < nu) {
-                s_v[w][i] = v[j];
-                s_x[w][i] = xv[j];
+            for (int j = 0; j < U2; j++) {
+                x0[j] = x[c[j][0]];
+                x1[j] = x[c[j][1]];
+            }
+#pragma unroll
+            for (int j = 0; j < U2; j++) {
+                const uint32_t q = base + j * WAVE + lane, g = pb + 2 * q;
+                if (q < np && g >= d.ubeg) {
+                    s_v[w][g - d.ubeg] = v[j][0];
+                    s_x[w][g - d.ubeg] = x0[j];
+                }
+                if (q < np && g + 1 < d.uend) {
+                    s_v[w][g + 1 - d.ubeg] = v[j][1];
+                    s_x[w][g + 1 - d.ubeg] = x1[j];
+                }
+            }
+        }
+    } else {
+        for (uint32_t base = 0; base < nu; base += WAVE * U) {
+            double v[U];
+            uint32_t c[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                uint32_t i = base + j * WAVE + lane;
+                i = i < nu ? i : nu - 1; // branch-free: re-read the last entry
+                v[j] = ntload(val + d.ubeg + i);
+                c[j] = ntload(col + d.ubeg + i);
+            }
+            double xv[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) xv[j] = x[c[j]];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                uint32_t i = base + j * WAVE + lane;
+                if (i < nu) {
+                    s_v[w][i] = v[j];
+                    s_x[w][i] = xv[j];
+                }
             }
         }
     }
@@ -142,9 +195,15 @@ extern "C" int spmv_csr(intspmv_ctx *ctx, const intspmv_plan *plan, const double
     if (st) return st;
     if (plan->nblocks == 0) return INTSPMV_OK;
     if (!row || !x || !y || (plan->nunits && (!val || !col))) return INTSPMV_ERR_ARG;
-    csr_block_kernel<<<ispv_grid(plan), BLOCK_THREADS, 0, ctx->stream>>>(
-        plan->d_blocks, plan->nlong_wg, plan->nblocks, val, col, row, x, y, plan->nrows,
-        plan->d_partial);
+    const bool pairs = plan->nunits >= 2 && (uintptr_t)val % 16 == 0 && (uintptr_t)col % 8 == 0;
+    if (pairs)
+        csr_block_kernel<true><<<ispv_grid(plan), BLOCK_THREADS, 0, ctx->stream>>>(
+            plan->d_blocks, plan->nlong_wg, plan->nblocks, val, col, row, x, y, plan->nrows,
+            plan->d_partial, plan->nunits);
+    else
+        csr_block_kernel<false><<<ispv_grid(plan), BLOCK_THREADS, 0, ctx->stream>>>(
+            plan->d_blocks, plan->nlong_wg, plan->nblocks, val, col, row, x, y, plan->nrows,
+            plan->d_partial, plan->nunits);
     st = ispv_launch_check(ctx);
     if (st) return st;
     return ispv_split_fixup(plan, y, 0);

@@ -84,6 +84,56 @@ def test_csr_mats(engine, mats):
         assert bitwise_equal(g2[short], yr[short]), name
 
 
+def _small_csr(kind):
+    rng = np.random.default_rng(11)
+    if kind == "one":
+        return H.CSR.from_arrays(np.array([0, 1], np.uint32), np.array([0], np.uint32),
+                                 np.array([2.5]))
+    if kind == "three":
+        return H.CSR.from_arrays(np.array([0, 1, 2, 3], np.uint32),
+                                 np.array([0, 2, 1], np.uint32), np.array([1.5, -2.0, 0.25]))
+    # odd entry count, rows of 0..40 entries, a long row in the last block
+    n = 1001
+    lens = rng.integers(0, 40, size=n)
+    lens[-1] = 301
+    if lens.sum() % 2 == 0:
+        lens[3] += 1
+    row = np.zeros(n + 1, np.uint32)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.integers(0, n, size=l)) for l in lens]).astype(np.uint32)
+    return H.CSR.from_arrays(row, col, rng.standard_normal(len(col)))
+
+
+@pytest.mark.parametrize("kind", ["one", "three", "odd"])
+def test_csr_block_odd_entry_counts(engine, kind):
+    """Pairwise loads never read past the arrays; the lone last entry of an
+    odd count is substituted; unaligned user arrays take the one-entry path."""
+    import ctypes as C
+
+    import torch
+    m = _small_csr(kind)
+    assert m.nnz % 2 == 1
+    x = H.uniform(m.nrows, seed=5)
+    yr = O.cf_csr(m.row, m.col, m.val, x)
+    d = D.DeviceCSR(engine, m)
+    xd, y = engine.upload(x), _y(engine, m.nrows)
+    d.spmv(xd, y)
+    g = _host(y, m.nrows)
+    short = np.diff(m.row.astype(np.int64)) <= 256
+    assert bitwise_equal(g[short], yr[short]) and scaled(m, x, yr, g) <= TOL64
+    # the same arrays one element into a larger buffer: 8-byte aligned values
+    vbuf = torch.zeros(m.nnz + 1, dtype=torch.float64, device=engine.torch_device)
+    cbuf = torch.zeros(m.nnz + 1, dtype=torch.int32, device=engine.torch_device)
+    vbuf[1:] = d.val
+    cbuf[1:] = d.col
+    y2 = _y(engine, m.nrows)
+    st = D.lib().spmv_csr(engine.h, d.plan.h, C.c_void_p(vbuf.data_ptr() + 8),
+                          C.c_void_p(cbuf.data_ptr() + 4), D._dptr(d.row), D._dptr(xd),
+                          D._dptr(y2))
+    assert st == 0
+    assert bitwise_equal(_host(y2, m.nrows), g)
+
+
 # ---------------------------------------------------------------- AXC / K1
 
 
