@@ -20,7 +20,10 @@
 
 namespace ispv {
 
-constexpr uint32_t AXC_UMAX = 128;
+// Blocks of 64 bricks (8 KB, all in flight at once), rows above 48 bricks on
+// waves of their own; matrices with many empty rows take 128 / 64.
+constexpr uint32_t AXC_UMAX = 128; // LDS capacity (bricks per block)
+constexpr uint32_t AXC_UMAX_DENSE = 64, AXC_LONG_DENSE = 48;
 constexpr uint32_t AXC_RMAX = 256;
 constexpr uint32_t AXC_LONG = 64;
 constexpr uint32_t K1_WIDE = 32;   // chunks with more steps get a whole wave
@@ -60,16 +63,24 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
         finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial);
         return;
     }
-    for (uint32_t base = 0; base < nu; base += 8 * U) {
-        d2v v[U];
+    // this lane's first row bounds, fetched before the stream
+    const uint32_t r0 = d.rbeg + lane;
+    uint32_t ra = 0, rb = 0;
+    if (r0 < d.rend) {
+        ra = brp[r0];
+        rb = brp[r0 + 1];
+    }
+    constexpr int UN = AXC_UMAX_DENSE / 8; // a dense-matrix block in flight at once
+    for (uint32_t base = 0; base < nu; base += 8 * UN) {
+        d2v v[UN];
 #pragma unroll
-        for (int j = 0; j < U; j++) {
+        for (int j = 0; j < UN; j++) {
             uint32_t t = base + j * 8 + lane / 8;
             size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
             v[j] = ntload(reinterpret_cast<const d2v *>(ax + T * 16 + 2 * p));
         }
 #pragma unroll
-        for (int j = 0; j < U; j++) {
+        for (int j = 0; j < UN; j++) {
             const bool ok = base + j * 8 + lane / 8 < nu && p < 4;
             double x0 = dpp_f64<DPP_ROW_SHL4>(v[j][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[j][1]);
             double p0 = ok ? v[j][0] * x0 : 0.0, p1 = ok ? v[j][1] * x1 : 0.0;
@@ -82,8 +93,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
         }
     }
     wave_lds_sync();
-    for (uint32_t r = d.rbeg + lane; r < d.rend; r += WAVE) {
-        uint32_t a = brp[r] / 16 - d.ubeg, b = brp[r + 1] / 16 - d.ubeg;
+    for (uint32_t r = r0; r < d.rend; r += WAVE) {
+        uint32_t a = (r == r0 ? ra : brp[r]) / 16 - d.ubeg, b = (r == r0 ? rb : brp[r + 1]) / 16 - d.ubeg;
         double sum = 0.0;
         for (uint32_t i = a; i < b; i++) sum = sum + s_val[w][i];
         y[r] = sum;
@@ -211,8 +222,9 @@ extern "C" int intspmv_axc_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t
         }
         uptr[r] = brp_host[r] / 16;
     }
-    int st = ispv_plan_build(ctx, PLAN_AXC, nrows, uptr, AXC_UMAX, AXC_RMAX, AXC_LONG, 1024,
-                             plan);
+    const bool sparse_rows = ispv_many_empty_rows(uptr, nrows);
+    int st = ispv_plan_build(ctx, PLAN_AXC, nrows, uptr, sparse_rows ? AXC_UMAX : AXC_UMAX_DENSE,
+                             AXC_RMAX, sparse_rows ? AXC_LONG : AXC_LONG_DENSE, 1024, plan);
     free(uptr);
     return st;
 }

@@ -28,7 +28,7 @@ namespace ispv {
 // FP64 reference layout, th = 1: blocks of 64 tiles (8 KB), all in flight at
 // once, and rows above 48 tiles on waves of their own (measured against 128 /
 // 64: 85.3 -> 83.8 us on the power-law matrix, 173.7 -> 168.0 us on the
-// stencil).
+// stencil); matrices with many empty rows keep 128 / 64.
 #ifndef ISPV_H1_F64_UMAX
 #define ISPV_H1_F64_UMAX 64
 #endif
@@ -37,9 +37,6 @@ namespace ispv {
 #endif
 #ifndef ISPV_H1_RED_UMAX
 #define ISPV_H1_RED_UMAX 128 // th = 1, FP32 / bf16 storage and the column-map form
-#endif
-#ifndef ISPV_H1_F64_STEPS
-#define ISPV_H1_F64_STEPS (ISPV_H1_F64_UMAX / 16)
 #endif
 constexpr uint32_t AXT_UMAX = ISPV_AXT_UMAX; // units (tiles or columns) per block
 constexpr uint32_t AXT_RMAX = 256;  // rows per block
@@ -282,10 +279,11 @@ __device__ __forceinline__ void stage_keys(const uint32_t *sec, const BlockDesc 
     if (nu == 0) return;
     const int lane = lane_id();
     uint32_t kk[UMAX / WAVE];
+    kk[0] = ntload(sec + d.ubeg + (lane < (int)nu ? lane : nu - 1));
 #pragma unroll
-    for (int j = 0; j < (int)(UMAX / WAVE); j++) {
+    for (int j = 1; j < (int)(UMAX / WAVE); j++) {
         const uint32_t t = j * WAVE + lane;
-        kk[j] = ntload(sec + d.ubeg + (t < nu ? t : nu - 1));
+        if (j * WAVE < (int)nu) kk[j] = ntload(sec + d.ubeg + (t < nu ? t : nu - 1)); // uniform
     }
 #pragma unroll
     for (int j = 0; j < (int)(UMAX / WAVE); j++)
@@ -320,10 +318,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
         // FP64 reference layout: each load instruction covers 8 whole 128-byte
         // tiles (1 KB, full cache lines).  Lane part p = l%8 holds values
         // 2p,2p+1 (p<4) or x 2(p-4),2(p-4)+1 (p>=4); one DPP row shift brings
-        // the x pair to the value lanes, whose quads then form the tree.
+        // the x pair to the value lanes, whose quads then form the tree.  A
+        // 64-tile block is in flight at once (8 loads).
         const uint32_t p = lane & 7;
-        constexpr int U = 2 * ISPV_H1_F64_STEPS;
-        stage_keys<ISPV_H1_F64_UMAX>(sec, d, nu, s_key[w]);
+        constexpr int U = ISPV_H1_F64_UMAX / 8;
+        stage_keys(sec, d, nu, s_key[w]);
         for (uint32_t base = 0; base < nu; base += 8 * U) {
             d2v v[U];
 #pragma unroll
@@ -984,8 +983,9 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
     // block size and long-row threshold per kernel (see the kernels' notes)
     const bool h1_f64 = th == 1 && D.layout == INTSPMV_AXT_INTERLEAVED &&
                         D.val_type == INTSPMV_F64 && D.x_type == INTSPMV_F64;
-    const uint32_t umax = h1_f64 ? ISPV_H1_F64_UMAX : th == 1 ? ISPV_H1_RED_UMAX : AXT_UMAX;
-    uint32_t long_units = th == 1 ? (h1_f64 ? ISPV_H1_F64_LONG : ISPV_H1_LONG)
+    const bool small = h1_f64 && !ispv_many_empty_rows(uptr, D.nrows);
+    const uint32_t umax = small ? ISPV_H1_F64_UMAX : th == 1 ? ISPV_H1_RED_UMAX : AXT_UMAX;
+    uint32_t long_units = th == 1 ? (small ? ISPV_H1_F64_LONG : ISPV_H1_LONG)
                                   : (512 / th < 4 ? 4 : (512 / th > umax / 2 ? umax / 2 : 512 / th));
     int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, umax, AXT_RMAX, long_units,
                              LONG_SLOTS / ths * (th == 1 ? 1 : 8), plan);

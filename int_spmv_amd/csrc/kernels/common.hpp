@@ -183,6 +183,15 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
                     intspmv_plan **out);
 #include <vector>
 // host-only block builder (abi.hip), shared by the single-matrix and batch plans
+// True when more than a quarter of the rows own no unit (uptr[r] == uptr[r+1]).
+// Such matrices amortise the per-block row bookkeeping (row ranges, zero
+// fill) better with larger blocks.
+inline bool ispv_many_empty_rows(const uint32_t *uptr, uint32_t nrows)
+{
+    uint64_t empty = 0;
+    for (uint32_t r = 0; r < nrows; r++) empty += uptr[r + 1] == uptr[r];
+    return empty * 4 > nrows;
+}
 int ispv_build_blocks(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
                       uint32_t long_units, uint32_t long_max, std::vector<ispv::BlockDesc> &blocks,
                       std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg, uint32_t &nlong,

@@ -147,7 +147,7 @@ def test_axc(engine, lap256, mats, tiny10):
         d.int_axc(y)
         g = _host(y, m.nrows)
         ref = O.int_axc(m.nrows, a.ax, a.brp)
-        long_rows = np.diff(a.brp.astype(np.int64)) // 16 > 64
+        long_rows = np.diff(a.brp.astype(np.int64)) // 16 > 48  # rows on waves of their own
         assert bitwise_equal(g[~long_rows], ref[~long_rows]), name
         assert scaled(m, x, O.cf_csr(m.row, m.col, m.val, x), g) <= TOL64, name
 
