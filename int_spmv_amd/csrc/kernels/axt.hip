@@ -35,6 +35,9 @@ namespace ispv {
 #ifndef ISPV_H1_F64_LONG
 #define ISPV_H1_F64_LONG 48
 #endif
+#ifndef ISPV_TH8_UMAX
+#define ISPV_TH8_UMAX 64 // th = 8, FP64 and FP32 reference layouts
+#endif
 #ifndef ISPV_H1_RED_UMAX
 #define ISPV_H1_RED_UMAX 128 // th = 1, FP32 / bf16 storage and the column-map form
 #endif
@@ -699,7 +702,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl32_kernel(
     LaneSum<float> lacc; // long rows
     if (d.k == 0) stage_keys(sec, d, nu, s_key[w]);
     constexpr int L = TH / 2;           // lines per tile
-    constexpr int G = L >= 8 ? 1 : 2;   // tile groups of 8 tiles per step (block = 16 tiles)
+    constexpr int G = L >= 4 ? 1 : 2;   // tile groups of 8 tiles per step (th 8: 8-tile blocks)
     for (uint32_t Tb = T0; Tb < T1; Tb += 8 * G) {
         float4 v[G][L];
 #pragma unroll
@@ -983,8 +986,13 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
     // block size and long-row threshold per kernel (see the kernels' notes)
     const bool h1_f64 = th == 1 && D.layout == INTSPMV_AXT_INTERLEAVED &&
                         D.val_type == INTSPMV_F64 && D.x_type == INTSPMV_F64;
-    const bool small = h1_f64 && !ispv_many_empty_rows(uptr, D.nrows);
-    const uint32_t umax = small ? ISPV_H1_F64_UMAX : th == 1 ? ISPV_H1_RED_UMAX : AXT_UMAX;
+    // Smaller blocks keep a whole block in one load step where that measured
+    // faster (FP64 th = 1: 64 tiles; th = 8: 64 columns = 8 KB); matrices with
+    // many empty rows keep 128-unit blocks.
+    const bool dense_rows = !ispv_many_empty_rows(uptr, D.nrows);
+    const bool small = h1_f64 && dense_rows;
+    const uint32_t umax = small ? ISPV_H1_F64_UMAX : th == 1 ? ISPV_H1_RED_UMAX
+                        : (th == 8 && dense_rows) ? ISPV_TH8_UMAX : AXT_UMAX;
     uint32_t long_units = th == 1 ? (small ? ISPV_H1_F64_LONG : ISPV_H1_LONG)
                                   : (512 / th < 4 ? 4 : (512 / th > umax / 2 ? umax / 2 : 512 / th));
     int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, umax, AXT_RMAX, long_units,
