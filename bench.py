@@ -373,6 +373,17 @@ def main():
         w, e = timed(lambda: g.matvec(xd, y), 20, 3, torch)
         fwd_g = e
         record("AXT_GATHER_H1_F64_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC), w, e)
+        # conversion: a new x into the embedded AXT arrays, on the device
+        # (spmv_axt_refresh_x) versus the host builder (getFormatDataAXT)
+        t0 = time.perf_counter()
+        H.AXT(m, x, th=1)
+        host_build = time.perf_counter() - t0
+        dacol = eng.upload(ag.acol)  # same slots as the headline's arrays
+        w, e = timed(lambda: D.refresh_x(d1, dacol, xd), 20, 3, torch)
+        formats["AXT_X_REFRESH_H1"] = {"us": round(e * 1e6, 2),
+                                       "host_build_ms": round(host_build * 1e3, 1)}
+        log(f"[bench] AXT_X_REFRESH_H1: {formats['AXT_X_REFRESH_H1']}")
+        del dacol
         k8 = 8
         X8 = torch.rand(n * k8, dtype=torch.float64, device=eng.torch_device)
         Y8 = eng.empty(n * k8)
@@ -406,18 +417,28 @@ def main():
             else:
                 record(f"ROCSPARSE_CSR_F64_{alg}", M.bytes_csr(n, nnz), t, t,
                        {"rel_err_vs_csr": rel})
-        del dc
         c = H.AXC(m, x)
         dcc = D.DeviceAXC(eng, c)
         w, e = timed(lambda: dcc.int_axc(y), 20, 3, torch)
         record("AXC_F64", M.bytes_axc(n, c.lenAX), w, e)
         del dcc
         hosts["AXC"] = c
+        t0 = time.perf_counter()
         k = H.K1(m)
+        host_build = time.perf_counter() - t0
         dk = D.DeviceK1(eng, k)
         w, e = timed(lambda: dk.int_k1p(xd, y), 20, 3, torch)
-        record("K1P_F64", M.bytes_k1(n, k.lenVC, k.chunkNum), w, e)
-        del dk
+        # the K1 conversion on the device from the device CSR (intspmv_k1_build_device)
+        D.DeviceK1.from_device_csr(eng, dc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dk2 = D.DeviceK1.from_device_csr(eng, dc)
+        torch.cuda.synchronize()
+        dev_build = time.perf_counter() - t0
+        record("K1P_F64", M.bytes_k1(n, k.lenVC, k.chunkNum), w, e,
+               {"build_host_ms": round(host_build * 1e3, 1),
+                "build_device_ms": round(dev_build * 1e3, 2)})
+        del dk, dk2, dc
         hosts["K1"] = k
 
     batch = None

@@ -97,6 +97,18 @@ int spmv_k1(intspmv_ctx *ctx, const intspmv_plan *plan, const double *val, const
 /* orderArrayK1 alone: y[permi[i]] = res[i] for i < nrows. */
 int spmv_k1_order(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *permi, const double *res,
                   double *y);
+/* The K1 conversion on the device from a device CSR (row[nrows+1], col, val):
+ * getFormatDataK1 with orderFunction, getArrayPermiK1, getArraysNmcChpK1 and
+ * getArraysValColK1 (intSpmv_main.c:775-905), the same arrays as the host
+ * builder (rows sorted by length, longest first, ties in row order).  Writes
+ * permi[(chunkNum+1)*8], nmc[chunkNum], chp[chunkNum] (chunkNum =
+ * ceil(nrows/8)) and *lenVC; with val_out/col_out non-NULL also the lenVC
+ * entries of val/col (INTSPMV_ERR_OVERFLOW if cap_vc < lenVC: call once with
+ * NULL to learn lenVC).  Synchronous; allocates temporaries. */
+int intspmv_k1_build_device(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row,
+                            const uint32_t *col, const double *val, uint32_t *permi,
+                            uint32_t *nmc, uint32_t *chp, double *val_out, uint32_t *col_out,
+                            uint32_t cap_vc, uint32_t *lenVC);
 
 /* ---- AXT ----------------------------------------------------------------
  * Replaces int_axt_noc_h1 (intSpmv_main.c:1385-1404), int_axt_noc

@@ -68,6 +68,8 @@ SIGNATURES = [
     ("intspmv_k1_plan", C.c_int, [_vp, _u32, _u32, _vp, C.POINTER(_vp)]),
     ("spmv_k1", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_k1_order", C.c_int, [_vp, _u32, _vp, _vp, _vp]),
+    ("intspmv_k1_build_device", C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                          _u32, C.POINTER(_u32)]),
     ("intspmv_axt_plan", C.c_int, [_vp, C.POINTER(AxtDesc), _vp, C.POINTER(_vp)]),
     ("spmv_axt", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_axt_gather", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32, _u32]),
@@ -254,6 +256,34 @@ class DeviceK1:
         eng._check(lib().intspmv_k1_plan(eng.h, k.nrows, k.chunkNum, _np_ptr(nmc), C.byref(h)),
                    "k1_plan")
         self.plan = _Plan(eng, h)
+
+    @classmethod
+    def from_device_csr(cls, eng: Engine, dc: "DeviceCSR") -> "DeviceK1":
+        """K1 arrays built on the device from a DeviceCSR (intspmv_k1_build_device)."""
+        torch = _torch()
+        self = cls.__new__(cls)
+        n = dc.nrows
+        self.eng, self.nrows, self.chunkNum = eng, n, (n + 7) // 8
+        i32 = dict(dtype=torch.int32, device=eng.torch_device)
+        self.permi = torch.empty((self.chunkNum + 1) * 8, **i32)
+        self.nmc = torch.empty(max(self.chunkNum, 1), **i32)
+        self.chp = torch.empty(max(self.chunkNum, 1), **i32)
+        lenvc = _u32()
+        args = (eng.h, n, _dptr(dc.row), _dptr(dc.col), _dptr(dc.val), _dptr(self.permi),
+                _dptr(self.nmc), _dptr(self.chp))
+        eng._check(lib().intspmv_k1_build_device(*args, None, None, 0, C.byref(lenvc)),
+                   "k1_build_device")
+        self.lenVC = lenvc.value
+        self.val = torch.empty(max(self.lenVC, 1), dtype=torch.float64, device=eng.torch_device)
+        self.col = torch.empty(max(self.lenVC, 1), **i32)
+        eng._check(lib().intspmv_k1_build_device(*args, _dptr(self.val), _dptr(self.col),
+                                                 self.lenVC, C.byref(lenvc)), "k1_build_device")
+        nmc = np.ascontiguousarray(self.nmc[: self.chunkNum].cpu().numpy().view(np.uint32))
+        h = _vp()
+        eng._check(lib().intspmv_k1_plan(eng.h, n, self.chunkNum, _np_ptr(nmc), C.byref(h)),
+                   "k1_plan")
+        self.plan = _Plan(eng, h)
+        return self
 
     def int_k1(self, x, res) -> None:
         """Permuted result res[chunkNum*8] (int_k1, intSpmv_main.c:943-965)."""
