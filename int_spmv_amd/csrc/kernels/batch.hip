@@ -47,6 +47,7 @@ struct intspmv_batch {
     ispv::BlockDesc *d_blocks;
     ispv::BatchSplit *d_splits;
     double *d_partial;
+    uint32_t *d_xcd; // item ranges of the XCDs: [xcd[i], xcd[i+1])
     hipGraph_t graph;
     hipGraphExec_t exec;
     hipStream_t cap_stream;
@@ -57,14 +58,20 @@ namespace ispv {
 
 template <typename VT, typename XT, typename AT>
 __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
-    const BatchItem *__restrict__ items, uint32_t nitems, const BlockDesc *__restrict__ blocks,
-    const BatchMat *__restrict__ mats, double *partial)
+    const BatchItem *__restrict__ items, const uint32_t *__restrict__ xcd_range,
+    const BlockDesc *__restrict__ blocks, const BatchMat *__restrict__ mats, double *partial)
 {
     __shared__ GatherLds<AT, 8> lds[WAVES];
     const uint32_t w = threadIdx.x / WAVE;
-    const uint32_t total = gridDim.x * WAVES;
-    for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + w); it < nitems;
-         it += total) {
+    // Workgroups are dealt round-robin to the XCDs (grid a multiple of
+    // NUM_XCD): the waves of XCD x sweep its own contiguous item range
+    // together, so neighbouring blocks of a matrix -- which gather the same
+    // rows of x -- run at the same time on the L2 they share.
+    const uint32_t xcd = blockIdx.x % NUM_XCD;
+    const uint32_t total = gridDim.x / NUM_XCD * WAVES, end = xcd_range[xcd + 1];
+    for (uint32_t it = __builtin_amdgcn_readfirstlane(xcd_range[xcd] +
+                                                      (blockIdx.x / NUM_XCD) * WAVES + w);
+         it < end; it += total) {
         const BatchItem item = items[it];
         const BatchMat md = mats[item.mat];
         const BlockDesc d = blocks[item.block];
@@ -173,11 +180,24 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
         for (const SplitDesc &sp : ms) splits.push_back({i, sp.row, sp.b0, sp.k});
         piece_base += npieces;
     }
-    // longest items first; static striding then gives every wave a similar mix
-    std::stable_sort(cand.begin(), cand.end(),
-                     [](const Cand &a, const Cand &b) { return a.cost > b.cost; });
+    // items stay in (matrix, block, first RHS) order; the XCDs get contiguous
+    // ranges of equal cost
     std::vector<BatchItem> items(cand.size());
-    for (size_t i = 0; i < cand.size(); i++) items[i] = cand[i].item;
+    uint64_t cost_all = 0;
+    for (size_t i = 0; i < cand.size(); i++) {
+        items[i] = cand[i].item;
+        cost_all += cand[i].cost;
+    }
+    std::vector<uint32_t> xcd_range(NUM_XCD + 1, (uint32_t)cand.size());
+    xcd_range[0] = 0;
+    {
+        uint64_t acc = 0;
+        int x = 1;
+        for (size_t i = 0; i < cand.size() && x < NUM_XCD; i++) {
+            acc += cand[i].cost;
+            while (x < NUM_XCD && acc * NUM_XCD >= cost_all * (uint64_t)x) xcd_range[x++] = (uint32_t)(i + 1);
+        }
+    }
 
     intspmv_batch *b = (intspmv_batch *)calloc(1, sizeof(intspmv_batch));
     if (!b) return INTSPMV_ERR_NOMEM;
@@ -211,6 +231,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
     if (per_cu < 1) per_cu = 1;
     uint32_t want = (b->nitems + WAVES - 1) / WAVES;
     b->grid = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)(cus * per_cu), want));
+    b->grid = (b->grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD; // whole rounds over the XCDs
     auto up = [&](void **dst, const void *src, size_t bytes) {
         if (e != hipSuccess) return;
         e = hipMalloc(dst, bytes ? bytes : 16);
@@ -218,6 +239,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
     };
     up((void **)&b->d_mats, hm.data(), hm.size() * sizeof(BatchMat));
     up((void **)&b->d_items, items.data(), items.size() * sizeof(BatchItem));
+    up((void **)&b->d_xcd, xcd_range.data(), xcd_range.size() * sizeof(uint32_t));
     up((void **)&b->d_blocks, blocks.data(), blocks.size() * sizeof(BlockDesc));
     up((void **)&b->d_splits, splits.data(), splits.size() * sizeof(BatchSplit));
     if (e == hipSuccess)
@@ -237,13 +259,13 @@ static int batch_enqueue(const intspmv_batch *b, hipStream_t s)
     if (b->nitems) {
         if (b->val_type == INTSPMV_F64)
             batch_kernel<double, double, double><<<b->grid, BLOCK_THREADS, 0, s>>>(
-                b->d_items, b->nitems, b->d_blocks, b->d_mats, b->d_partial);
+                b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);
         else if (b->val_type == INTSPMV_F32)
             batch_kernel<float, float, float><<<b->grid, BLOCK_THREADS, 0, s>>>(
-                b->d_items, b->nitems, b->d_blocks, b->d_mats, b->d_partial);
+                b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);
         else
             batch_kernel<uint16_t, float, float><<<b->grid, BLOCK_THREADS, 0, s>>>(
-                b->d_items, b->nitems, b->d_blocks, b->d_mats, b->d_partial);
+                b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);
     }
     if (b->nsplit) {
         dim3 g((b->nsplit * PARTIAL_STRIDE + 255) / 256);
@@ -321,6 +343,7 @@ extern "C" int intspmv_batch_destroy(intspmv_batch *b)
     (void)hipFree(b->d_blocks);
     (void)hipFree(b->d_splits);
     (void)hipFree(b->d_partial);
+    (void)hipFree(b->d_xcd);
     free(b);
     return INTSPMV_OK;
 }
