@@ -49,38 +49,47 @@ __device__ __forceinline__ double gfma<double>(double a, double b, double c) { r
 template <>
 __device__ __forceinline__ float gfma<float>(float a, float b, float c) { return fmaf(a, b, c); }
 
-// x values of one slot for KC right-hand sides starting at j0 (0 for padding).
-// When the row offset c*ldx + j0 is a multiple of the vector width the KC
-// values come in 16-byte loads.
+// x values of one slot for KC right-hand sides starting at j0 (0 for padding),
+// for KC > 1 without per-lane branches: a padding slot reads row 0 and its
+// values are masked, so the loads of a lane's slots issue back to back.  When every
+// value is wanted (kv == KC) and the row offset c*ldx + j0 is a multiple of
+// the vector width (uniform), the KC values come in 16-byte loads; otherwise
+// one load per value, indexes clamped to the row's first kv values.
 template <typename XT, typename AT, int KC>
 __device__ __forceinline__ void gather_x(const XT *x, uint32_t c, uint32_t ldx, uint32_t j0,
                                          uint32_t kv, AT (&xv)[KC])
 {
     constexpr int VW = 16 / sizeof(XT); // elements per 16-byte load
-    if constexpr (KC >= VW) {
-        if (c != PAD && kv == (uint32_t)KC && ((ldx | j0) % VW) == 0) {
-            const XT *p = x + (size_t)c * ldx + j0;
+    const bool ok = c != PAD;
+    const XT *p = x + (size_t)(ok ? c : 0) * ldx + j0;
+    if constexpr (KC >= VW && KC % VW == 0) {
+        if (kv == (uint32_t)KC && ((ldx | j0) % VW) == 0) {
 #pragma unroll
             for (int j = 0; j < KC; j += VW) {
                 if constexpr (sizeof(XT) == 8) {
                     d2v v = *reinterpret_cast<const d2v *>(p + j);
-                    xv[j] = (AT)v[0];
-                    xv[j + 1] = (AT)v[1];
+                    xv[j] = ok ? (AT)v[0] : (AT)0;
+                    xv[j + 1] = ok ? (AT)v[1] : (AT)0;
                 } else {
                     typedef float f4v __attribute__((ext_vector_type(4)));
                     f4v v = *reinterpret_cast<const f4v *>(p + j);
-                    xv[j] = (AT)v[0];
-                    xv[j + 1] = (AT)v[1];
-                    xv[j + 2] = (AT)v[2];
-                    xv[j + 3] = (AT)v[3];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) xv[j + e] = ok ? (AT)v[e] : (AT)0;
                 }
             }
             return;
         }
     }
+    if constexpr (KC == 1) {
+        // one value: a masked load, so padding slots issue no request
+        xv[0] = (ok && kv) ? (AT)p[0] : (AT)0;
+    } else {
 #pragma unroll
-    for (int j = 0; j < KC; j++)
-        xv[j] = (c != PAD && (uint32_t)j < kv) ? (AT)x[(size_t)c * ldx + j0 + j] : (AT)0;
+        for (int j = 0; j < KC; j++) {
+            const XT v = p[(uint32_t)j < kv ? j : 0];
+            xv[j] = (ok && (uint32_t)j < kv) ? (AT)v : (AT)0;
+        }
+    }
 }
 
 // One pass over KC right-hand sides.  th == 1: four lanes per tile, the
@@ -161,6 +170,7 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                     continue;
                 }
                 const uint32_t key = ntload(sec + T);
+                AT ts[KC];
 #pragma unroll
                 for (int j = 0; j < KC; j++) {
                     AT p0 = a0[u][j], p1 = a1[u][j];
@@ -168,9 +178,13 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                     p1 = p1 + dpp_any<AT, DPP_QUAD_XOR2>(p1);
                     p0 = p0 + dpp_any<AT, DPP_QUAD_XOR1>(p0);
                     p1 = p1 + dpp_any<AT, DPP_QUAD_XOR1>(p1);
-                    if (ok && q == 0) sv[(Tr - d.ubeg) * KC + j] = p0 + p1;
+                    ts[j] = p0 + p1;
                 }
-                if (ok && q == 0) sk[Tr - d.ubeg] = key;
+                if (ok && q == 0) { // one branch for the tile's KC sums and its key
+#pragma unroll
+                    for (int j = 0; j < KC; j++) sv[(Tr - d.ubeg) * KC + j] = ts[j];
+                    sk[Tr - d.ubeg] = key;
+                }
             } else {
                 const uint32_t u0 = 8 * Tr + C;
                 const bool ok0 = Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
@@ -184,13 +198,16 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                     continue;
                 }
                 const u2v kk = ntload(reinterpret_cast<const u2v *>(sec + 8 * T + C));
+                if (ok0) { // one branch per column for its KC sums and key
 #pragma unroll
-                for (int j = 0; j < KC; j++) {
-                    if (ok0) sv[(u0 - d.ubeg) * KC + j] = a0[u][j];
-                    if (ok1) sv[(u0 + 1 - d.ubeg) * KC + j] = a1[u][j];
+                    for (int j = 0; j < KC; j++) sv[(u0 - d.ubeg) * KC + j] = a0[u][j];
+                    sk[u0 - d.ubeg] = kk[0];
                 }
-                if (ok0) sk[u0 - d.ubeg] = kk[0];
-                if (ok1) sk[u0 + 1 - d.ubeg] = kk[1];
+                if (ok1) {
+#pragma unroll
+                    for (int j = 0; j < KC; j++) sv[(u0 + 1 - d.ubeg) * KC + j] = a1[u][j];
+                    sk[u0 + 1 - d.ubeg] = kk[1];
+                }
             }
         }
     }
@@ -212,12 +229,17 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
     const uint32_t nseg = find_segments(sk, sg, nu);
     for (uint32_t si = lane; si < nseg; si += WAVE) {
         const uint32_t a = sg[si], b = sg[si + 1], r = sk[a];
-        for (int j = 0; j < KC; j++) {
-            if ((uint32_t)j >= kv) break;
-            AT acc = 0;
-            for (uint32_t i = a; i < b; i++) acc = acc + sv[i * KC + j];
-            if (r < nrows) y[(size_t)r * ldy + j0 + j] = acc;
-        }
+        // all KC sums of the row at once (units in order for each of them)
+        AT acc[KC];
+#pragma unroll
+        for (int j = 0; j < KC; j++) acc[j] = 0;
+        for (uint32_t i = a; i < b; i++)
+#pragma unroll
+            for (int j = 0; j < KC; j++) acc[j] = acc[j] + sv[i * KC + j];
+        if (r < nrows)
+#pragma unroll
+            for (int j = 0; j < KC; j++)
+                if ((uint32_t)j < kv) y[(size_t)r * ldy + j0 + j] = acc[j];
     }
     zero_empty_rows(sk, sg, nseg, d, nrows, [&](uint32_t z) {
         for (uint32_t j = 0; j < kv; j++) y[(size_t)z * ldy + j0 + j] = (AT)0;
