@@ -32,7 +32,7 @@ def parse():
     p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
-    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--workload", default="powerlaw_1m")
     p.add_argument("--quick", action="store_true", help="headline kernel only (profiling)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")

@@ -35,6 +35,10 @@ namespace ispv {
 #ifndef ISPV_H1_F64_LONG
 #define ISPV_H1_F64_LONG 48
 #endif
+#ifndef ISPV_TH2_UMAX
+#define ISPV_TH2_UMAX 256 // th = 2, FP64: 256 columns = 8 KB per block
+#endif
+constexpr uint32_t TH2_UMAX = ISPV_TH2_UMAX;
 #ifndef ISPV_TH8_UMAX
 #define ISPV_TH8_UMAX 64 // th = 8, FP64 and FP32 reference layouts
 #endif
@@ -278,7 +282,7 @@ template <uint32_t UMAX = AXT_UMAX>
 __device__ __forceinline__ void stage_keys(const uint32_t *sec, const BlockDesc &d, uint32_t nu,
                                            uint32_t *sk)
 {
-    static_assert(UMAX % WAVE == 0 && UMAX <= AXT_UMAX, "key staging");
+    static_assert(UMAX % WAVE == 0, "key staging");
     if (nu == 0) return;
     const int lane = lane_id();
     uint32_t kk[UMAX / WAVE];
@@ -605,9 +609,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
     const double *__restrict__ ax, const uint32_t *__restrict__ sec, double *__restrict__ y,
     uint32_t nrows, double *partial, uint32_t th_rt)
 {
-    __shared__ double s_val[WAVES][AXT_UMAX];
-    __shared__ uint32_t s_key[WAVES][AXT_UMAX];
-    __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
+    // th = 2 blocks hold 256 columns (8 KB, like th = 4's 128): capacity per height
+    constexpr uint32_t UM = TH == 2 ? TH2_UMAX : AXT_UMAX;
+    __shared__ double s_val[WAVES][UM];
+    __shared__ uint32_t s_key[WAVES][UM];
+    __shared__ uint16_t s_seg[WAVES][UM + 1];
     const uint32_t th = TH ? TH : th_rt;
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const uint32_t bid = block_index(nlong_wg);
@@ -616,10 +622,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
     const uint32_t nu = d.uend - d.ubeg, p = lane & 7, C = 2 * (p & 3);
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
     double lacc = 0.0; // long rows
-    if (d.k == 0) stage_keys(sec, d, nu, s_key[w]); // unit u = column 8T+C has key sec[u]
-    // G groups of 8 tiles per step: a normal block is 16 tiles, so two
-    // groups put th = 2 and 4 blocks in flight at once
-    constexpr int G = (TH == 0 || TH >= 8) ? 1 : 2;
+    if (d.k == 0) stage_keys<UM>(sec, d, nu, s_key[w]); // unit u = column 8T+C has key sec[u]
+    // G groups of 8 tiles per step, so that a normal block is in flight at
+    // once: th = 2: 32 tiles; th = 4: 16 tiles (th = 8, 16: 8 tiles per step)
+    constexpr int G = TH == 2 ? (int)(TH2_UMAX / 64) : (TH == 0 || TH >= 8) ? 1 : 2;
     for (uint32_t Tb = T0; Tb < T1; Tb += 8 * G) {
         double a0[G], a1[G];
         if constexpr (TH > 0) {
@@ -991,7 +997,10 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
     // many empty rows keep 128-unit blocks.
     const bool dense_rows = !ispv_many_empty_rows(uptr, D.nrows);
     const bool small = h1_f64 && dense_rows;
+    const bool fl64 = D.layout == INTSPMV_AXT_INTERLEAVED && D.val_type == INTSPMV_F64 &&
+                      D.x_type == INTSPMV_F64 && D.mode == INTSPMV_NOC;
     const uint32_t umax = small ? ISPV_H1_F64_UMAX : th == 1 ? ISPV_H1_RED_UMAX
+                        : (th == 2 && fl64) ? TH2_UMAX
                         : (th == 8 && dense_rows) ? ISPV_TH8_UMAX : AXT_UMAX;
     uint32_t long_units = th == 1 ? (small ? ISPV_H1_F64_LONG : ISPV_H1_LONG)
                                   : (512 / th < 4 ? 4 : (512 / th > umax / 2 ? umax / 2 : 512 / th));

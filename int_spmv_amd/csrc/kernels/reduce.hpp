@@ -134,11 +134,7 @@ __device__ __forceinline__ void reduce_keyed(const AT *sv, const uint32_t *sk, u
         AT acc = 0;
         for (uint32_t i = a; i < b; i++) acc = acc + sv[i];
         uint32_t r = sk[a];
-#ifdef ISPV_NT_Y
-        if (r < nrows) __builtin_nontemporal_store(acc, y + r);
-#else
         if (r < nrows) y[r] = acc;
-#endif
     }
     zero_empty_rows(sk, sseg, nseg, d, nrows, [&](uint32_t z) { y[z] = (AT)0; });
 }
