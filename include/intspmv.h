@@ -124,15 +124,19 @@ typedef struct {
     int32_t mode;     /* INTSPMV_NOC or INTSPMV_COM */
     int32_t val_type; /* stored value type: F64, F32 or BF16 */
     int32_t x_type;   /* embedded / gathered x type: F64 or F32 */
-    int32_t layout;   /* INTSPMV_AXT_INTERLEAVED or INTSPMV_AXT_PLANAR */
+    int32_t layout;   /* INTSPMV_AXT_INTERLEAVED, _PLANAR or _COLMAP */
 } intspmv_axt_desc;
 
 /* INTERLEAVED: the reference's ax: per brick-row 8 values then 8 x, both of
  *   the same type (F64/F64 is the drop-in; F32/F32 the FP32 storage variant).
  * PLANAR: values and embedded x in two separate slot-ordered arrays (slot s of
  *   tile T, brick-row R, lane C is T*tileH*8 + R*8 + C); used for mixed types
- *   such as BF16 values with FP32 x. */
-enum { INTSPMV_AXT_INTERLEAVED = 0, INTSPMV_AXT_PLANAR = 1 };
+ *   such as BF16 values with FP32 x.
+ * COLMAP: a plan for the column-map entry points below (spmv_axt_gather,
+ *   spmv_axt_grad_vals); spmv_axt refuses it.  Plans of the embedded layouts
+ *   may use larger blocks than the column-map kernels stage and are then
+ *   refused by spmv_axt_gather. */
+enum { INTSPMV_AXT_INTERLEAVED = 0, INTSPMV_AXT_PLANAR = 1, INTSPMV_AXT_COLMAP = 2 };
 
 /* con_host: str_matAXT.con (lenCON = nrows) from the builder. */
 int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc, const uint32_t *con_host,
@@ -146,8 +150,8 @@ int spmv_axt(intspmv_ctx *ctx, const intspmv_plan *plan, const void *ax, const v
 /* ---- AXT with a column map (x not embedded) -----------------------------
  * aval[s] / acol[s]: value and column of slot s in the tile order above
  * (acol = 0xFFFFFFFF on padding), sec as built.  The plan is the AXT plan of
- * the same matrix (its val_type / x_type select the kernel; layout is not
- * used).  X is n x k row-major with leading dimension ldx >= k, Y likewise;
+ * the same matrix, built with layout INTSPMV_AXT_COLMAP (its val_type /
+ * x_type select the kernel).  X is n x k row-major with leading dimension ldx >= k, Y likewise;
  * 1 <= k <= 32 right-hand sides.  Y = A X.  No reference counterpart: the
  * reference embeds x (intSpmv_main.c:1171, 1207), which fixes one x per build. */
 int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval,

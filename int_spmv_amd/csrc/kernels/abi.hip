@@ -334,6 +334,7 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
     intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));
     if (!p) return INTSPMV_ERR_NOMEM;
     p->ctx = ctx;
+    p->device = ctx->device;
     p->kind = kind;
     p->nrows = nrows;
     p->nunits = nrows ? uptr[nrows] : 0;
@@ -358,7 +359,7 @@ int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *
 extern "C" int intspmv_plan_destroy(intspmv_plan *p)
 {
     if (!p) return INTSPMV_OK;
-    if (p->ctx) ispv_enter(p->ctx);
+    (void)hipSetDevice(p->device); // not through the context: it may be gone already
     if (p->d_blocks) (void)hipFree(p->d_blocks);
     if (p->d_partial) (void)hipFree(p->d_partial);
     if (p->d_splits) (void)hipFree(p->d_splits);

@@ -39,6 +39,15 @@ namespace ispv {
 #define ISPV_TH2_UMAX 256 // th = 2, FP64: 256 columns = 8 KB per block
 #endif
 constexpr uint32_t TH2_UMAX = ISPV_TH2_UMAX;
+// FP32 and bf16 embedded-x storage at th = 2, 4: blocks of RED_TH_UMAX columns
+// (256: 4-8 KB per block rather than 2-4 KB); the kernels size their LDS for it.
+#ifndef ISPV_RED_TH_UMAX
+#define ISPV_RED_TH_UMAX 256
+#endif
+__host__ __device__ constexpr uint32_t red_umax(int th)
+{
+    return (th == 2 || th == 4) ? ISPV_RED_TH_UMAX : ISPV_AXT_UMAX;
+}
 #ifndef ISPV_TH8_UMAX
 #define ISPV_TH8_UMAX 64 // th = 8, FP64 and FP32 reference layouts
 #endif
@@ -696,9 +705,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl32_kernel(
     uint32_t nrows, double *partial)
 {
     static_assert(TH >= 2 && TH % 2 == 0, "brick-row pairs");
-    __shared__ float s_val[WAVES][AXT_UMAX];
-    __shared__ uint32_t s_key[WAVES][AXT_UMAX];
-    __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
+    constexpr uint32_t UM = red_umax(TH); // th 2, 4: 256-column blocks
+    __shared__ float s_val[WAVES][UM];
+    __shared__ uint32_t s_key[WAVES][UM];
+    __shared__ uint16_t s_seg[WAVES][UM + 1];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
@@ -706,9 +716,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl32_kernel(
     const uint32_t nu = d.uend - d.ubeg, p = lane & 7;
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
     LaneSum<float> lacc; // long rows
-    if (d.k == 0) stage_keys(sec, d, nu, s_key[w]);
-    constexpr int L = TH / 2;           // lines per tile
-    constexpr int G = L >= 4 ? 1 : 2;   // tile groups of 8 tiles per step (th 8: 8-tile blocks)
+    if (d.k == 0) stage_keys<UM>(sec, d, nu, s_key[w]);
+    constexpr int L = TH / 2; // lines per tile
+    // tile groups of 8 tiles per step: the whole block in flight (th 8: 8-tile blocks)
+    constexpr int G = L >= 4 ? 1 : (int)(UM / 64);
     for (uint32_t Tb = T0; Tb < T1; Tb += 8 * G) {
         float4 v[G][L];
 #pragma unroll
@@ -792,9 +803,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_planar_kernel(
     float *__restrict__ y, uint32_t nrows, double *partial)
 {
     static_assert(TH == 1 || TH == 2 || TH == 4 || TH == 8 || TH == 16, "tile height");
-    __shared__ float s_val[WAVES][AXT_UMAX];
-    __shared__ uint32_t s_key[WAVES][AXT_UMAX];
-    __shared__ uint16_t s_seg[WAVES][AXT_UMAX + 1];
+    constexpr uint32_t UM = red_umax(TH);
+    __shared__ float s_val[WAVES][UM];
+    __shared__ uint32_t s_key[WAVES][UM];
+    __shared__ uint16_t s_seg[WAVES][UM + 1];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
@@ -803,8 +815,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_planar_kernel(
     const size_t T0 = TH == 1 ? d.ubeg : d.ubeg / 8, T1 = TH == 1 ? d.uend : (d.uend + 7) / 8;
     const size_t H0 = T0 * 2 * TH, H1 = T1 * 2 * TH;
     LaneSum<float> lacc; // long rows
-    if (d.k == 0) stage_keys(sec, d, nu, s_key[w]);
-    constexpr int U = TH == 1 ? ISPV_PLANAR_STEPS : ISPV_PLANAR_STEPS_TH;
+    if (d.k == 0) stage_keys<UM>(sec, d, nu, s_key[w]);
+    // load steps of 64 half-brick-rows: th 2, 4 take their whole block at once
+    constexpr int U = TH == 1 ? ISPV_PLANAR_STEPS
+                    : (TH == 2 || TH == 4) ? (int)(UM * TH / 256) : ISPV_PLANAR_STEPS_TH;
     for (size_t hb = H0; hb < H1; hb += 64 * U) {
         float4 v[U], x[U];
 #pragma unroll
@@ -940,6 +954,9 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
     const intspmv_axt_desc &D = *desc;
     if (D.tileHW != 8 || D.tileH == 0) return INTSPMV_ERR_ARG;
     if (D.mode != INTSPMV_NOC && D.mode != INTSPMV_COM) return INTSPMV_ERR_ARG;
+    if (D.layout != INTSPMV_AXT_INTERLEAVED && D.layout != INTSPMV_AXT_PLANAR &&
+        D.layout != INTSPMV_AXT_COLMAP)
+        return INTSPMV_ERR_ARG;
     if (D.mode == INTSPMV_COM) {
         // units are slots = CSR positions; con[r] = (p/8)*16 + p%8 with p the
         // first CSR position of row r (getArraysLenAXT with positions = rl)
@@ -997,11 +1014,18 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
     // many empty rows keep 128-unit blocks.
     const bool dense_rows = !ispv_many_empty_rows(uptr, D.nrows);
     const bool small = h1_f64 && dense_rows;
+    // the embedded-x kernels only (column-map plans keep blocks of at most
+    // AXT_UMAX units, what the column-map kernels stage)
     const bool fl64 = D.layout == INTSPMV_AXT_INTERLEAVED && D.val_type == INTSPMV_F64 &&
                       D.x_type == INTSPMV_F64 && D.mode == INTSPMV_NOC;
+    const bool fl32 = D.layout == INTSPMV_AXT_INTERLEAVED && D.val_type == INTSPMV_F32 &&
+                      D.x_type == INTSPMV_F32 && D.mode == INTSPMV_NOC;
+    const bool planar32 = D.layout == INTSPMV_AXT_PLANAR && D.val_type != INTSPMV_F64 &&
+                          D.x_type == INTSPMV_F32 && D.mode == INTSPMV_NOC;
     const uint32_t umax = small ? ISPV_H1_F64_UMAX : th == 1 ? ISPV_H1_RED_UMAX
                         : (th == 2 && fl64) ? TH2_UMAX
-                        : (th == 8 && dense_rows) ? ISPV_TH8_UMAX : AXT_UMAX;
+                        : (th == 8 && dense_rows) ? ISPV_TH8_UMAX
+                        : (fl32 || planar32) ? red_umax(th) : AXT_UMAX;
     uint32_t long_units = th == 1 ? (small ? ISPV_H1_F64_LONG : ISPV_H1_LONG)
                                   : (512 / th < 4 ? 4 : (512 / th > umax / 2 ? umax / 2 : 512 / th));
     int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, umax, AXT_RMAX, long_units,
@@ -1020,6 +1044,7 @@ extern "C" int spmv_axt(intspmv_ctx *ctx, const intspmv_plan *plan, const void *
     if (plan->nrows && (!y || (plan->nunits && (!ax || !sec)))) return INTSPMV_ERR_ARG;
     int st = ispv_enter(ctx);
     if (st) return st;
+    if (D.layout == INTSPMV_AXT_COLMAP) return INTSPMV_ERR_ARG; // a column-map plan
     const bool planar = D.layout == INTSPMV_AXT_PLANAR;
     if (planar && plan->nunits && !xplane) return INTSPMV_ERR_ARG;
     if (!planar) {

@@ -40,6 +40,7 @@ struct BatchSplit {
 
 struct intspmv_batch {
     intspmv_ctx *ctx;
+    int device; // the context's device, so destroying needs nothing of the context
     int val_type, x_type;
     uint32_t nmat, nitems, nblocks, nsplit, grid;
     ispv::BatchMat *d_mats;
@@ -202,6 +203,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
     intspmv_batch *b = (intspmv_batch *)calloc(1, sizeof(intspmv_batch));
     if (!b) return INTSPMV_ERR_NOMEM;
     b->ctx = ctx;
+    b->device = ctx->device;
     b->val_type = val_type;
     b->x_type = x_type;
     b->nmat = nmat;
@@ -334,7 +336,7 @@ extern "C" int intspmv_batch_info(const intspmv_batch *b, uint32_t *nitems, uint
 extern "C" int intspmv_batch_destroy(intspmv_batch *b)
 {
     if (!b) return INTSPMV_OK;
-    if (b->ctx) ispv_enter(b->ctx);
+    (void)hipSetDevice(b->device); // not through the context: it may be gone already
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);

@@ -158,6 +158,7 @@ enum PlanKind { PLAN_CSR = 1, PLAN_AXC = 2, PLAN_AXT = 3, PLAN_K1 = 4 };
 
 struct intspmv_plan {
     intspmv_ctx *ctx;
+    int device; // the context's device, so destroying needs nothing of the context
     int kind;
     uint32_t nrows, nunits, nblocks, nsplit_rows, nlong_rows;
     uint32_t nlong_wg;  // workgroups of the long-row section (multiple of 8)

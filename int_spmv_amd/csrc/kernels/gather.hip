@@ -173,8 +173,10 @@ extern "C" int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const
                                const uint32_t *acol, const uint32_t *sec, const void *x,
                                uint32_t ldx, void *y, uint32_t ldy, uint32_t k, uint32_t kc)
 {
-    if (!gather_plan_ok(ctx, plan) || k == 0 || k > PARTIAL_STRIDE || ldx < k || ldy < k ||
-        (kc != 0 && kc != 1 && kc != 2 && kc != 4 && kc != 8))
+    // the column-map blocks stage at most G_UMAX units: a plan built for the
+    // reference-layout kernels with larger blocks is refused
+    if (!gather_plan_ok(ctx, plan) || plan->umax > G_UMAX || k == 0 || k > PARTIAL_STRIDE ||
+        ldx < k || ldy < k || (kc != 0 && kc != 1 && kc != 2 && kc != 4 && kc != 8))
         return INTSPMV_ERR_ARG;
     if (plan->nrows && (!x || !y || (plan->nunits && (!aval || !acol || !sec))))
         return INTSPMV_ERR_ARG;

@@ -18,6 +18,7 @@ built library or a GPU the constructors raise.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -25,7 +26,7 @@ from ._loader import load_library
 
 F64, F32, BF16 = 0, 1, 2
 NOC, COM = 0, 1
-INTERLEAVED, PLANAR = 0, 1
+INTERLEAVED, PLANAR, COLMAP = 0, 1, 2
 
 _vp = C.c_void_p
 _u32 = C.c_uint32
@@ -186,7 +187,9 @@ class Engine:
         return torch.empty(max(n, 1), dtype=tdt, device=self.torch_device)
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
+        # at interpreter exit the HIP runtime and the other objects go in no
+        # fixed order: leave the memory to the process teardown
+        if getattr(self, "h", None) and _lib is not None and not sys.is_finalizing():
             _lib.intspmv_destroy(self.h)
             self.h = None
 
@@ -201,7 +204,7 @@ class _Plan:
         return dict(nblocks=a.value, nsplit_rows=b.value, nlong_rows=c.value)
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
+        if getattr(self, "h", None) and _lib is not None and not sys.is_finalizing():
             _lib.intspmv_plan_destroy(self.h)
             self.h = None
 
@@ -388,7 +391,7 @@ class DeviceAXTGather:
         self.eng, self.nrows, self.a = eng, a.nrows, a
         self.val_type, self.x_type = val_type, x_type
         self.nslots = a.lenAX // 2
-        self.desc = _axt_desc(a, val_type, x_type, PLANAR)
+        self.desc = _axt_desc(a, val_type, x_type, COLMAP)
         self.aval = vals if vals is not None else eng.upload(_value_plane(a, val_type))
         self.acol = eng.upload(a.acol)
         self.sec = eng.upload(a.sec)
@@ -523,6 +526,6 @@ class DeviceBatch:
         self.eng._check(lib().intspmv_batch_graph_launch(self.eng.h, self.h), "batch_replay")
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
+        if getattr(self, "h", None) and _lib is not None and not sys.is_finalizing():
             _lib.intspmv_batch_destroy(self.h)
             self.h = None
