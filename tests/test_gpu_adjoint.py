@@ -169,3 +169,24 @@ def test_autograd_finite_differences(engine):
             scale = ((gv * dv).abs().sum() + (gx * dx).abs().sum()).item()
             worst = max(worst, abs(fd - an) / scale)
     assert worst < 1e-6, worst
+
+
+def test_plan_layouts_are_checked(engine):
+    """A column-map plan is refused by the embedded product, and an embedded
+    plan with larger blocks than the column-map kernels stage is refused by
+    the column-map product (INTSPMV_ERR_ARG), instead of running."""
+    import ctypes as C
+
+    import torch
+    m = H.CSR.powerlaw(n=4096, target_nnz=60_000, maxlen=512, seed=2)
+    a = H.AXT(m, H.uniform(m.nrows, seed=1), th=2, want_acol=True)
+    g = D.DeviceAXTGather(engine, a)
+    e = D.DeviceAXT(engine, a)  # FP64 reference layout, th = 2: 256-column blocks
+    y = torch.empty(m.nrows, dtype=torch.float64, device=engine.torch_device)
+    x = engine.upload(H.uniform(m.nrows, seed=3))
+    st = D.lib().spmv_axt(engine.h, g.plan.h, D._dptr(e.ax), None, D._dptr(e.sec), D._dptr(y))
+    assert st == -1
+    st = D.lib().spmv_axt_gather(engine.h, e.plan.h, D._dptr(g.aval), D._dptr(g.acol),
+                                 D._dptr(g.sec), D._dptr(x), 1, D._dptr(y), 1, 1, 0)
+    assert st == -1
+    assert e.plan.info()["nblocks"] > 0 and C.sizeof(D.AxtDesc) > 0
