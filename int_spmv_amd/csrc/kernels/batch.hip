@@ -43,6 +43,7 @@ struct intspmv_batch {
     int device; // the context's device, so destroying needs nothing of the context
     int val_type, x_type;
     uint32_t nmat, nitems, nblocks, nsplit, grid;
+    bool xb; // every x spans less than 2 GiB: buffer-resource gathers
     ispv::BatchMat *d_mats;
     ispv::BatchItem *d_items;
     ispv::BlockDesc *d_blocks;
@@ -57,7 +58,7 @@ struct intspmv_batch {
 
 namespace ispv {
 
-template <typename VT, typename XT, typename AT>
+template <typename VT, typename XT, typename AT, bool XB>
 __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
     const BatchItem *__restrict__ items, const uint32_t *__restrict__ xcd_range,
     const BlockDesc *__restrict__ blocks, const BatchMat *__restrict__ mats, double *partial)
@@ -85,10 +86,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
 #define ISPV_B(KC)                                                                              \
     if (md.kc == KC) {                                                                          \
         if (md.th == 1)                                                                         \
-            gather_block<VT, XT, AT, KC, true>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy,  \
+            gather_block<VT, XT, AT, KC, true, XB>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy,  \
                                                item.j0, kv, 1, md.nrows, part, L.val, L.key, L.seg); \
         else                                                                                    \
-            gather_block<VT, XT, AT, KC, false>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy, \
+            gather_block<VT, XT, AT, KC, false, XB>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy, \
                                                 item.j0, kv, md.th, md.nrows, part, L.val, L.key,  \
                                                 L.seg);                                         \
     }
@@ -141,6 +142,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
     };
     std::vector<Cand> cand;
     uint32_t piece_base = 0;
+    bool xb = true;
     for (uint32_t i = 0; i < nmat; i++) {
         const intspmv_batch_matrix &M = mats[i];
         const uint32_t th = M.tileH;
@@ -170,6 +172,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
                                    th == 1 ? 1024 : 8192 / th, mb, ms, nlw, nlong, npieces);
         if (st) return st;
         hm[i] = {M.aval, M.acol, M.sec, M.x, M.y, M.nrows, th, M.k, M.ldx, M.ldy, kc, piece_base, 0};
+        xb = xb && xbuf_fits(M.nrows, M.ldx, x_type == INTSPMV_F64 ? 8 : 4);
         for (const BlockDesc &d : mb) {
             const uint32_t bi = (uint32_t)blocks.size();
             blocks.push_back(d);
@@ -217,14 +220,18 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
     }
     int dev = ctx->device, cus = 0, per_cu = 0;
     hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    b->xb = xb;
+#define ISPV_OCC(VT, XT, AT)                                                                    \
+    (xb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<VT, XT, AT, true>,  \
+                                                       BLOCK_THREADS, 0)                         \
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<VT, XT, AT, false>, \
+                                                       BLOCK_THREADS, 0))
     if (e == hipSuccess) {
-        if (f64)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<double, double, double>, BLOCK_THREADS, 0);
-        else if (f32)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<float, float, float>, BLOCK_THREADS, 0);
-        else
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<uint16_t, float, float>, BLOCK_THREADS, 0);
+        if (f64) e = ISPV_OCC(double, double, double);
+        else if (f32) e = ISPV_OCC(float, float, float);
+        else e = ISPV_OCC(uint16_t, float, float);
     }
+#undef ISPV_OCC
     if (e != hipSuccess || cus <= 0) {
         ctx->last_err = e;
         free(b);
@@ -259,15 +266,21 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
 static int batch_enqueue(const intspmv_batch *b, hipStream_t s)
 {
     if (b->nitems) {
-        if (b->val_type == INTSPMV_F64)
-            batch_kernel<double, double, double><<<b->grid, BLOCK_THREADS, 0, s>>>(
-                b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);
-        else if (b->val_type == INTSPMV_F32)
-            batch_kernel<float, float, float><<<b->grid, BLOCK_THREADS, 0, s>>>(
-                b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);
-        else
-            batch_kernel<uint16_t, float, float><<<b->grid, BLOCK_THREADS, 0, s>>>(
-                b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);
+#define ISPV_BL(VT, XT, AT)                                                                     \
+    if (b->xb)                                                                                  \
+        batch_kernel<VT, XT, AT, true><<<b->grid, BLOCK_THREADS, 0, s>>>(                        \
+            b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);                         \
+    else                                                                                        \
+        batch_kernel<VT, XT, AT, false><<<b->grid, BLOCK_THREADS, 0, s>>>(                       \
+            b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial)
+        if (b->val_type == INTSPMV_F64) {
+            ISPV_BL(double, double, double);
+        } else if (b->val_type == INTSPMV_F32) {
+            ISPV_BL(float, float, float);
+        } else {
+            ISPV_BL(uint16_t, float, float);
+        }
+#undef ISPV_BL
     }
     if (b->nsplit) {
         dim3 g((b->nsplit * PARTIAL_STRIDE + 255) / 256);

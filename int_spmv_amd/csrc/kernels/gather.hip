@@ -20,7 +20,7 @@
 
 namespace ispv {
 
-template <typename VT, typename XT, typename AT, int KC, bool H1>
+template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
     const VT *__restrict__ aval, const uint32_t *__restrict__ acol,
@@ -33,7 +33,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
     GatherLds<AT, KC> &L = lds[threadIdx.x / WAVE];
-    gather_block<VT, XT, AT, KC, H1>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv, th, nrows,
+    gather_block<VT, XT, AT, KC, H1, XB>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv, th, nrows,
                                      partial, L.val, L.key, L.seg);
 }
 
@@ -132,26 +132,39 @@ static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t
     dim3 grid(ispv_grid(p)), blk(BLOCK_THREADS);
     hipStream_t s = p->ctx->stream;
     const uint32_t nl = p->nlong_wg;
+    // buffer-resource gathers for passes of fewer right-hand sides than one
+    // 16-byte load holds (with more, the plain-load kernel measured faster)
+    const bool xb_ok = xbuf_fits(p->nrows, ldx, sizeof(XT));
     for (uint32_t j0 = 0; j0 < k;) {
         const uint32_t rem = k - j0;
         const uint32_t kc = kcf ? kcf : (rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1)));
         const uint32_t kv = rem < kc ? rem : kc;
+        const bool xb = xb_ok && kc < 16 / sizeof(XT);
+#define ISPV_GX(KC, H1, XB)                                                                     \
+    axt_gather_kernel<VT, XT, AT, KC, H1, XB><<<grid, blk, 0, s>>>(                             \
+        p->d_blocks, nl, nb, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y, ldy, j0, \
+        kv, th, p->nrows, p->d_partial)
 #define ISPV_G(KC)                                                                              \
     if (kc == KC) {                                                                             \
-        if (th == 1)                                                                            \
-            axt_gather_kernel<VT, XT, AT, KC, true><<<grid, blk, 0, s>>>(                       \
-                p->d_blocks, nl, nb, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y,  \
-                ldy, j0, kv, th, p->nrows, p->d_partial);                                       \
-        else                                                                                    \
-            axt_gather_kernel<VT, XT, AT, KC, false><<<grid, blk, 0, s>>>(                      \
-                p->d_blocks, nl, nb, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y,  \
-                ldy, j0, kv, th, p->nrows, p->d_partial);                                       \
+        if constexpr (KC < 16 / sizeof(XT)) {                                                   \
+            if (th == 1) {                                                                      \
+                if (xb) ISPV_GX(KC, true, true);                                                \
+                else ISPV_GX(KC, true, false);                                                  \
+            } else {                                                                            \
+                if (xb) ISPV_GX(KC, false, true);                                               \
+                else ISPV_GX(KC, false, false);                                                 \
+            }                                                                                   \
+        } else {                                                                                \
+            if (th == 1) ISPV_GX(KC, true, false);                                              \
+            else ISPV_GX(KC, false, false);                                                     \
+        }                                                                                       \
     }
         ISPV_G(1)
         ISPV_G(2)
         ISPV_G(4)
         ISPV_G(8)
 #undef ISPV_G
+#undef ISPV_GX
         int st = ispv_launch_check(p->ctx);
         if (st) return st;
         j0 += kc;

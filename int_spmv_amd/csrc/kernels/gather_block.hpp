@@ -49,24 +49,43 @@ __device__ __forceinline__ double gfma<double>(double a, double b, double c) { r
 template <>
 __device__ __forceinline__ float gfma<float>(float a, float b, float c) { return fmaf(a, b, c); }
 
+// Single x values may be read through a buffer resource when x spans less
+// than 2 GiB (XB, chosen on the host: xbuf_fits): a padding slot then loads
+// from an offset past the resource's range, which the hardware answers with
+// zeros and no memory request.  Otherwise, and for the 16-byte loads of
+// several right-hand sides (measured faster as plain loads), padding reads
+// row 0 and is masked.
+constexpr uint32_t XB_OOB = 0x80000000u; // past every range xbuf_fits accepts
+constexpr int XB_DWORD3 = 0x00020000;    // buffer resource word 3 for gfx9-family raw loads
+__host__ __device__ inline bool xbuf_fits(uint32_t nrows, uint32_t ldx, uint32_t xsize)
+{
+    return (uint64_t)nrows * ldx * xsize < XB_OOB;
+}
+template <typename XT>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_xbuf(const XT *x, uint32_t nrows,
+                                                            uint32_t ldx)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<XT *>(x), 0,
+                                             (int)((uint64_t)nrows * ldx * sizeof(XT)), XB_DWORD3);
+}
+
 // x values of one slot for KC right-hand sides starting at j0 (0 for padding),
-// for KC > 1 without per-lane branches: a padding slot reads row 0 and its
-// values are masked, so the loads of a lane's slots issue back to back.  When every
-// value is wanted (kv == KC) and the row offset c*ldx + j0 is a multiple of
-// the vector width (uniform), the KC values come in 16-byte loads; otherwise
-// one load per value, indexes clamped to the row's first kv values.
-template <typename XT, typename AT, int KC>
-__device__ __forceinline__ void gather_x(const XT *x, uint32_t c, uint32_t ldx, uint32_t j0,
-                                         uint32_t kv, AT (&xv)[KC])
+// without per-lane branches.  When every value is wanted (kv == KC) and the row
+// offset c*ldx + j0 is a multiple of the vector width (uniform), the KC values
+// come in 16-byte loads; otherwise one load per value.
+template <typename XT, typename AT, int KC, bool XB>
+__device__ __forceinline__ void gather_x(const XT *x, __amdgpu_buffer_rsrc_t xr, uint32_t c,
+                                         uint32_t ldx, uint32_t j0, uint32_t kv, AT (&xv)[KC])
 {
     constexpr int VW = 16 / sizeof(XT); // elements per 16-byte load
+    constexpr uint32_t SZ = sizeof(XT);
     const bool ok = c != PAD;
     const XT *p = x + (size_t)(ok ? c : 0) * ldx + j0;
     if constexpr (KC >= VW && KC % VW == 0) {
         if (kv == (uint32_t)KC && ((ldx | j0) % VW) == 0) {
 #pragma unroll
             for (int j = 0; j < KC; j += VW) {
-                if constexpr (sizeof(XT) == 8) {
+                if constexpr (SZ == 8) {
                     d2v v = *reinterpret_cast<const d2v *>(p + j);
                     xv[j] = ok ? (AT)v[0] : (AT)0;
                     xv[j + 1] = ok ? (AT)v[1] : (AT)0;
@@ -80,10 +99,19 @@ __device__ __forceinline__ void gather_x(const XT *x, uint32_t c, uint32_t ldx, 
             return;
         }
     }
-    if constexpr (KC == 1) {
-        // one value: a masked load, so padding slots issue no request
-        xv[0] = (ok && kv) ? (AT)p[0] : (AT)0;
+    if constexpr (XB) {
+        // every real offset is below the range (< 2 GiB), padding's is past it
+        const uint32_t off = (c * ldx + j0) * SZ;
+#pragma unroll
+        for (int j = 0; j < KC; j++) {
+            const uint32_t o = (ok && (uint32_t)j < kv) ? off + j * SZ : XB_OOB;
+            if constexpr (SZ == 8)
+                xv[j] = (AT)__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(xr, o, 0, 0));
+            else
+                xv[j] = (AT)__int_as_float(__builtin_amdgcn_raw_buffer_load_b32(xr, o, 0, 0));
+        }
     } else {
+        // indexes clamped to the row's first kv values
 #pragma unroll
         for (int j = 0; j < KC; j++) {
             const XT v = p[(uint32_t)j < kv ? j : 0];
@@ -104,7 +132,7 @@ struct GatherLds {
     uint16_t seg[G_UMAX + 1];
 };
 
-template <typename VT, typename XT, typename AT, int KC, bool H1>
+template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB>
 __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__restrict__ aval,
                                              const uint32_t *__restrict__ acol,
                                              const uint32_t *__restrict__ sec,
@@ -116,6 +144,7 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
 {
     const int lane = lane_id();
     const uint32_t nu = d.uend - d.ubeg, q = lane & 3, C = 2 * q;
+    const __amdgpu_buffer_rsrc_t xr = make_xbuf(x, nrows, ldx); // unused unless XB
     AT lacc[KC];
 #pragma unroll
     for (int j = 0; j < KC; j++) lacc[j] = 0;
@@ -125,11 +154,19 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
     const uint32_t T0 = H1 ? d.ubeg : d.ubeg / 8, T1 = H1 ? d.uend : (d.uend + 7) / 8;
     for (uint32_t Tbb = T0; Tbb < T1; Tbb += 16 * UG) {
         AT a0[UG][KC], a1[UG][KC];
+        u2v kk[UG]; // row keys: the tile's (th == 1) or the lane's two columns'
 #pragma unroll
-        for (int u = 0; u < UG; u++)
+        for (int u = 0; u < UG; u++) {
 #pragma unroll
             for (int j = 0; j < KC; j++) a0[u][j] = a1[u][j] = 0;
-        for (uint32_t R = 0; R < th; R++) {
+            const uint32_t Tr = Tbb + 16 * u + lane / 4;
+            const size_t T = Tr < T1 ? Tr : T1 - 1;
+            if (H1) kk[u][0] = ntload(sec + T);
+            else kk[u] = ntload(reinterpret_cast<const u2v *>(sec + 8 * T + C));
+        }
+        // th == 1 (H1): one brick-row, known here so that the value loads
+        // issue with the column loads instead of after the gathers
+        for (uint32_t R = 0; R < (H1 ? 1u : th); R++) {
             GLoad<VT> v0[UG], v1[UG];
             u2v cc[UG];
 #pragma unroll
@@ -143,8 +180,8 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
 #pragma unroll
             for (int u = 0; u < UG; u++) {
                 AT x0[KC], x1[KC];
-                gather_x<XT, AT, KC>(x, cc[u][0], ldx, j0, kv, x0);
-                gather_x<XT, AT, KC>(x, cc[u][1], ldx, j0, kv, x1);
+                gather_x<XT, AT, KC, XB>(x, xr, cc[u][0], ldx, j0, kv, x0);
+                gather_x<XT, AT, KC, XB>(x, xr, cc[u][1], ldx, j0, kv, x1);
 #pragma unroll
                 for (int j = 0; j < KC; j++) {
                     if (H1) {
@@ -169,7 +206,7 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                         for (int j = 0; j < KC; j++) lacc[j] = lacc[j] + (a0[u][j] + a1[u][j]);
                     continue;
                 }
-                const uint32_t key = ntload(sec + T);
+                const uint32_t key = kk[u][0];
                 AT ts[KC];
 #pragma unroll
                 for (int j = 0; j < KC; j++) {
@@ -197,16 +234,15 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                     }
                     continue;
                 }
-                const u2v kk = ntload(reinterpret_cast<const u2v *>(sec + 8 * T + C));
                 if (ok0) { // one branch per column for its KC sums and key
 #pragma unroll
                     for (int j = 0; j < KC; j++) sv[(u0 - d.ubeg) * KC + j] = a0[u][j];
-                    sk[u0 - d.ubeg] = kk[0];
+                    sk[u0 - d.ubeg] = kk[u][0];
                 }
                 if (ok1) {
 #pragma unroll
                     for (int j = 0; j < KC; j++) sv[(u0 + 1 - d.ubeg) * KC + j] = a1[u][j];
-                    sk[u0 + 1 - d.ubeg] = kk[1];
+                    sk[u0 + 1 - d.ubeg] = kk[u][1];
                 }
             }
         }

@@ -487,8 +487,8 @@ class DeviceBatch:
     right-hand sides in one persistent launch (optionally a hipGraph).
 
     entries: list of dicts with keys a (host AXT built with want_acol), X and
-    Y (device tensors, n*k row-major), k and optionally kc, umax (units per
-    wave block) and dev (the (aval, acol, sec) device arrays of a, from
+    Y (device tensors, n*k row-major), k and optionally ldx (leading
+    dimension of X, default k), kc, umax (units per wave block) and dev (the (aval, acol, sec) device arrays of a, from
     upload_batch_arrays, so that entries can share them)."""
 
     def __init__(self, eng: Engine, entries, val_type: int = F32, x_type: int = F32):
@@ -502,7 +502,7 @@ class DeviceBatch:
             self._keep += [aval, acol, sec, con, e["X"], e["Y"]]
             k = e["k"]
             arr[i] = BatchMatrix(aval=aval.data_ptr(), acol=acol.data_ptr(), sec=sec.data_ptr(),
-                                 x=e["X"].data_ptr(), y=e["Y"].data_ptr(), ldx=k, ldy=k, k=k,
+                                 x=e["X"].data_ptr(), y=e["Y"].data_ptr(), ldx=e.get("ldx", k), ldy=k, k=k,
                                  kc=e.get("kc", 0), nrows=a.nrows, tileH=a.tileH, tileN=a.tileN,
                                  lenAX=a.lenAX, colN=a.colN, umax=e.get("umax", 0),
                                  con_host=con.ctypes.data)

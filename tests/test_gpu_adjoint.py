@@ -75,6 +75,33 @@ def test_gather_multi_rhs(engine, mats, k, th):
             assert scaled(m, X[:, j], yr, Yg[:, j]) <= 1e-12, (name, k, th, j)
 
 
+@pytest.mark.parametrize("vt", [D.F64, D.F32])
+def test_gather_wide_x_plain_loads(engine, mats, vt):
+    """x spanning 2 GiB or more (here through a large leading dimension) is
+    gathered with plain loads instead of the buffer resource; both give the
+    same bits, padding included."""
+    import torch
+    m = mats[0][1]
+    n = m.nrows
+    dt = torch.float64 if vt == D.F64 else torch.float32
+    wide = (1 << 31) // (n * (8 if vt == D.F64 else 4))  # n * ldx * size == 2 GiB
+    for th, k in ((1, 8), (1, 3), (4, 1), (4, 8)):
+        X = np.stack([H.uniform(n, seed=60 + j) for j in range(k)], axis=1)
+        a = H.AXT(m, np.zeros(n), th=th, want_acol=True)
+        g = D.DeviceAXTGather(engine, a, val_type=vt, x_type=vt)
+        Xp = torch.tensor(X.reshape(-1), dtype=dt, device=engine.torch_device)
+        Xw = torch.zeros((n, wide), dtype=dt, device=engine.torch_device)
+        Xw[:, :k] = Xp.view(n, k)
+        Y1 = torch.full((n * k,), float("nan"), dtype=dt, device=engine.torch_device)
+        Y2 = torch.full_like(Y1, float("nan"))
+        g.matvec(Xp, Y1, k=k)
+        g.matvec(Xw.view(-1), Y2, k=k, ldx=wide)
+        torch.cuda.synchronize()
+        assert bitwise_equal(Y1.cpu().numpy(), Y2.cpu().numpy()), (th, k)
+        del Xw
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("vt,xt,tol", [(D.F32, D.F32, 1e-6), (D.BF16, D.F32, 1e-2)])
 def test_gather_precision(engine, mats, vt, xt, tol):
     for name, m in mats:

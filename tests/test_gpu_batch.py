@@ -112,3 +112,27 @@ def test_batch_wave_mapping_is_exact(engine):
             assert scaled(m, X[:, j].copy(), yref[:, j], yb.reshape(-1, k)[:, j]) <= 1e-12, umax
         if umax == 128:
             assert bitwise_equal(yb, yr)
+
+
+def test_batch_wide_x_plain_loads(engine):
+    """A batch holding an x of 2 GiB or more gathers every matrix with plain
+    loads; the bits equal the buffer-resource launch."""
+    import torch
+    mats = [(m, k) for (m, k) in batch_256(seed=5, count=6, nmax=8000)]
+    ents = _entries(engine, mats, (1, 4), torch.float64)
+    b = D.DeviceBatch(engine, ents, D.F64, D.F64)
+    b.launch()
+    ref = [_y(e).copy() for e in ents]
+    e0 = ents[0]
+    n, k = e0["m"].nrows, e0["k"]
+    wide = -(-(1 << 31) // (n * 8))
+    Xw = torch.zeros((n, wide), dtype=torch.float64, device=engine.torch_device)
+    Xw[:, :k] = e0["X"].view(n, k)
+    ents2 = [dict(e, Y=torch.full_like(e["Y"], float("nan"))) for e in ents]
+    ents2[0] = dict(ents2[0], X=Xw.view(-1), ldx=wide)
+    b2 = D.DeviceBatch(engine, ents2, D.F64, D.F64)
+    b2.launch()
+    for i, e in enumerate(ents2):
+        assert bitwise_equal(_y(e).reshape(-1), ref[i].reshape(-1)), i
+    del b2, ents2, Xw
+    torch.cuda.empty_cache()
