@@ -245,6 +245,23 @@ def cpu_leg(m, x, a1, hosts, formats, y1, head_bytes, workload, quick):
         a4 = hosts["AXT_H4"]
         t, it, y = run(5, threads, 2.0, a4.tileN, 4, [a4.ax, a4.sec])
         rec("int_axt_noc th4", t, it, threads, M.bytes_axt(n, a4.lenAX, a4.lenSEC), y)
+    # the rest of the reference's tile-height sweep, and its COM kernel at
+    # three block sizes (the reference's COM defects included: its errors
+    # against CSR are reported, not hidden)
+    from int_spmv_amd import host as H
+    for th in (2, 8, 16):
+        a = H.AXT(m, x, th=th)
+        t, it, y = run(5, threads, 1.5, a.tileN, th, [a.ax, a.sec])
+        rec(f"int_axt_noc th{th}", t, it, threads, M.bytes_axt(n, a.lenAX, a.lenSEC), y)
+        del a
+    for bs in (128, 256, 512):
+        a = H.AXT(m, x, th=1, mode="COM", bs=bs)
+        bn = (a.tileN * a.tileHW + bs - 1) // bs
+        prm = np.array([bs, a.log, a.tileHW, a.lenAX, a.lenSEC], np.uint32)
+        t, it, y = run(6, threads, 1.5, bn, 1, [a.ax, a.sec, prm])
+        rec(f"int_axt_com_h1 bs{bs} (reference defects kept)", t, it, threads,
+            M.bytes_axt(n, a.lenAX, a.lenSEC), y)
+        del a
     if CMP.mkl_available():
         for thr in sorted({1, threads}):
             res = CMP.mkl_csr(m.row, m.col, m.val, x, thr, budget_s=1.5)
@@ -401,7 +418,23 @@ def main():
         w, e = timed(lambda: g.grad_vals(xd, yb, vb), 20, 3, torch)
         record("AXT_GRAD_VALS_H1_F64", (ag.lenAX // 2) * (4 + 8) + ag.lenSEC * 4 + 2 * n * 8, w, e,
                {"vs_forward_gather": round(e / fwd_g, 3)})
-        del g, adj, ag, X8, Y8, vb
+        del g, adj, X8, Y8, vb
+        # the same in FP32 storage (values and x FP32, FP32 sums)
+        g32 = D.DeviceAXTGather(eng, ag, D.F32, D.F32)
+        x32 = xd.float()
+        y32 = torch.empty(n, dtype=torch.float32, device=eng.torch_device)
+        w, e = timed(lambda: g32.matvec(x32, y32), 20, 3, torch)
+        fwd32 = e
+        record("AXT_GATHER_H1_F32_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, sv=4, sx=4, sy=4),
+               w, e)
+        adj32 = D.AxtAdjoint(eng, m, g32)
+        yb32 = yb.float()
+        xb32 = torch.empty(n, dtype=torch.float32, device=eng.torch_device)
+        w, e = timed(lambda: adj32.adjoint(yb32, xb32), 20, 3, torch)
+        at = adj32.at
+        record("AXT_ADJOINT_H1_F32", M.bytes_axt_gather(n, at.lenAX, at.lenSEC, sv=4, sx=4, sy=4),
+               w, e, {"vs_forward_gather": round(e / fwd32, 3)})
+        del g32, adj32, ag, x32, y32, yb32, xb32
         dc = D.DeviceCSR(eng, m)
         y = eng.empty(n)
         w, e = timed(lambda: dc.spmv(xd, y), 20, 3, torch)

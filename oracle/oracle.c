@@ -175,41 +175,60 @@ void orc_int_axt_noc(uint32_t tileN, uint32_t th, const double *ax, const uint32
  * skipped, and segment sums are prefix differences.  Tiles past tileN, which
  * the reference reads beyond the end of ax, are taken as zero; they only
  * feed prefix entries that no segment reads. */
-void orc_int_axt_com_h1(uint32_t bn, uint32_t bs, uint32_t log, uint32_t thw, uint32_t tn,
-                        uint32_t lenAX, uint32_t lenSEC, const double *ax, const uint32_t *sec,
-                        double *y)
+/* One COM block b: products of its tiles in blk1, in-tile prefix sums
+ * continued across the block's tiles in blk2, then every segment start of the
+ * block adds its segment sum to y (atomically when other blocks run at the
+ * same time). */
+static void com_block(uint32_t b, uint32_t bs, uint32_t log, uint32_t thw, uint32_t lenAX,
+                      uint32_t lenSEC, const double *ax, const uint32_t *sec, double *blk1,
+                      double *blk2, double *y, int atomic)
 {
     const uint32_t tpb = bs / thw, lbs = tpb * 2 * thw;
-    double *blk1 = (double *)calloc((size_t)bs, sizeof(double));
-    double *blk2 = (double *)calloc((size_t)bs, sizeof(double));
-    (void)tn;
-    for (uint32_t b = 0; b < bn; b++) {
-        double acu = 0.0;
-        for (uint32_t t = 0; t < tpb; t++) {
-            size_t p = (size_t)b * lbs + (size_t)t * 2 * thw;
-            double v[HB], r1[HB], r2[HB], r3[HB];
-            for (uint32_t i = 0; i < HB; i++)
-                v[i] = (p + HB + i < lenAX) ? ax[p + i] * ax[p + HB + i] : 0.0;
-            for (uint32_t i = 0; i < HB; i++) r1[i] = i >= 1 ? v[i] + v[i - 1] : v[i];
-            for (uint32_t i = 0; i < HB; i++) r2[i] = i >= 2 ? r1[i] + r1[i - 2] : r1[i];
-            for (uint32_t i = 0; i < HB; i++) r3[i] = i >= 4 ? r2[i] + r2[i - 4] : r2[i];
-            for (uint32_t i = 0; i < HB; i++) {
-                blk1[t * HB + i] = v[i];
-                blk2[t * HB + i] = r3[i] + acu;
-            }
-            acu = blk2[t * HB + HB - 1];
+    double acu = 0.0;
+    for (uint32_t t = 0; t < tpb; t++) {
+        size_t p = (size_t)b * lbs + (size_t)t * 2 * thw;
+        double v[HB], r1[HB], r2[HB], r3[HB];
+        for (uint32_t i = 0; i < HB; i++)
+            v[i] = (p + HB + i < lenAX) ? ax[p + i] * ax[p + HB + i] : 0.0;
+        for (uint32_t i = 0; i < HB; i++) r1[i] = i >= 1 ? v[i] + v[i - 1] : v[i];
+        for (uint32_t i = 0; i < HB; i++) r2[i] = i >= 2 ? r1[i] + r1[i - 2] : r1[i];
+        for (uint32_t i = 0; i < HB; i++) r3[i] = i >= 4 ? r2[i] + r2[i - 4] : r2[i];
+        for (uint32_t i = 0; i < HB; i++) {
+            blk1[t * HB + i] = v[i];
+            blk2[t * HB + i] = r3[i] + acu;
         }
-        uint32_t sp = b * bs, ep = (b + 1) * bs;
-        ep = ep > lenSEC ? lenSEC : ep;
-        for (uint32_t q = sp; q < ep; q++) {
-            uint32_t ro = sec[q];
-            if (ro != 0) {
-                uint32_t r = ro >> log, o = ro & (bs - 1), l = q - sp;
-                double val = blk2[l + o] - blk2[l] + blk1[l];
+        acu = blk2[t * HB + HB - 1];
+    }
+    uint32_t sp = b * bs, ep = (b + 1) * bs;
+    ep = ep > lenSEC ? lenSEC : ep;
+    for (uint32_t q = sp; q < ep; q++) {
+        uint32_t ro = sec[q];
+        if (ro != 0) {
+            uint32_t r = ro >> log, o = ro & (bs - 1), l = q - sp;
+            double val = blk2[l + o] - blk2[l] + blk1[l];
+            if (atomic) {
+#pragma omp atomic
+                y[r] += val;
+            } else {
                 y[r] = y[r] + val;
             }
         }
     }
+}
+
+/* int_axt_com_h1, intSpmv_main.c:1517-1566, restated faithfully including
+ * its defects: segments whose sec word is 0 (row 0 with offset 0) are
+ * skipped, and segment sums are prefix differences.  Tiles past tileN, which
+ * the reference reads beyond the end of ax, are taken as zero; they only
+ * feed prefix entries that no segment reads.  Blocks in order (serial). */
+void orc_int_axt_com_h1(uint32_t bn, uint32_t bs, uint32_t log, uint32_t thw, uint32_t tn,
+                        uint32_t lenAX, uint32_t lenSEC, const double *ax, const uint32_t *sec,
+                        double *y)
+{
+    double *blk1 = (double *)calloc((size_t)bs, sizeof(double));
+    double *blk2 = (double *)calloc((size_t)bs, sizeof(double));
+    (void)tn;
+    for (uint32_t b = 0; b < bn; b++) com_block(b, bs, log, thw, lenAX, lenSEC, ax, sec, blk1, blk2, y, 0);
     free(blk1);
     free(blk2);
 }
@@ -299,6 +318,25 @@ static void omp_axt(uint32_t tileN, uint32_t th, const double *ax, const uint32_
     }
 }
 
+/* int_axt_com_h1 over the host's threads: blocks dealt dynamically, each
+ * thread with its own block scratch, y updated atomically (as the
+ * reference's parallel loop, intSpmv_main.c:1532-1564).
+ * prm = {bs, log, thw, lenAX, lenSEC}. */
+static void omp_axt_com(uint32_t bn, const uint32_t *prm, const double *ax, const uint32_t *sec,
+                        double *y)
+{
+#pragma omp parallel
+    {
+        double *blk1 = (double *)calloc((size_t)prm[0], sizeof(double));
+        double *blk2 = (double *)calloc((size_t)prm[0], sizeof(double));
+#pragma omp for schedule(dynamic)
+        for (int64_t b = 0; b < (int64_t)bn; b++)
+            com_block((uint32_t)b, prm[0], prm[1], prm[2], prm[3], prm[4], ax, sec, blk1, blk2, y, 1);
+        free(blk1);
+        free(blk2);
+    }
+}
+
 double orc_bench(int kind, int threads, int iters, uint32_t n, uint32_t len, uint32_t th,
                  const void *a0, const void *a1, const void *a2, const void *a3,
                  const void *a4, const void *a5, double *y)
@@ -335,6 +373,9 @@ double orc_bench(int kind, int threads, int iters, uint32_t n, uint32_t len, uin
             break;
         case 5:
             omp_axt(len, th, (const double *)a0, (const uint32_t *)a1, y);
+            break;
+        case 6: /* len = bn, a2 = {bs, log, thw, lenAX, lenSEC} */
+            omp_axt_com(len, (const uint32_t *)a2, (const double *)a0, (const uint32_t *)a1, y);
             break;
         default:
             return -1.0;

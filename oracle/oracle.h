@@ -55,7 +55,8 @@ void orc_int_axt_com_h1(uint32_t bn, uint32_t bs, uint32_t log, uint32_t thw, ui
  * atomics into y), timed per call with y zeroed outside the timed region as
  * in test_* (intSpmv_main.c:1419-1427).  kind: 0 cf_CSR (serial),
  * 1 int_axc, 2 int_k1 (+orderArrayK1 untimed), 3 int_k1p, 4 int_axt_noc_h1,
- * 5 int_axt_noc.  Returns the mean seconds per call; y receives the last
+ * 5 int_axt_noc, 6 int_axt_com_h1 (a2 = {bs, log, thw, lenAX, lenSEC}, len =
+ * the block count).  Returns the mean seconds per call; y receives the last
  * result.  The pointer arguments a0..a5 follow the kernel's own order. */
 double orc_bench(int kind, int threads, int iters, uint32_t n, uint32_t len, uint32_t th,
                  const void *a0, const void *a1, const void *a2, const void *a3,

@@ -161,41 +161,47 @@ def test_refresh_x_on_device(engine, mats):
         assert scaled(m, x1, O.cf_csr(m.row, m.col, m.val, x1), _host(y, m.nrows)) <= 1e-12
 
 
-def test_autograd_finite_differences(engine):
+@pytest.mark.parametrize("vt,h,tol_adj,tol_fd", [(D.F64, 1e-6, 1e-12, 1e-6),
+                                                 (D.F32, 1e-3, 1e-6, 1e-4)])
+def test_autograd_finite_differences(engine, vt, h, tol_adj, tol_fd):
     """L(v, x) = ybar^T A(v) x is bilinear, so central differences along any
-    direction equal the directional derivative up to rounding."""
+    direction equal the directional derivative up to rounding (FP64 with
+    h = 1e-6, FP32 with h = 1e-3; a float32 numpy restatement of this case
+    stays at 3e-6 relative, so 1e-4 holds with margin)."""
     import torch
 
     from int_spmv_amd import autograd as AG
     m = H.CSR.powerlaw(n=1 << 12, target_nnz=80_000, maxlen=2000, seed=5)
     n = m.nrows
-    op = AG.AxtOperator(engine, m, th=1)
-    dev = engine.torch_device
+    op = AG.AxtOperator(engine, m, th=1, val_type=vt)
+    dev, dt = engine.torch_device, op.dtype
     v = op.initial_values().requires_grad_(True)
-    x = torch.tensor(H.uniform(n, seed=21), device=dev, requires_grad=True)
-    ybar = torch.tensor(H.normal(n, seed=22), device=dev)
+    xh = H.uniform(n, seed=21).astype(np.float32 if vt == D.F32 else np.float64)
+    x = torch.tensor(xh, device=dev, requires_grad=True)
+    ybar = torch.tensor(H.normal(n, seed=22), device=dev, dtype=dt)
     L = (AG.spmv(v, x, op) * ybar).sum()
     L.backward()
     gv, gx = v.grad.detach(), x.grad.detach()
-    # x_bar against the CSR transpose product
-    ref = O.csr_transpose_product(m.row, m.col, m.val, ybar.cpu().numpy())
+    # x_bar against the CSR transpose product (of the values as stored)
+    vals = m.val.astype(np.float32).astype(np.float64) if vt == D.F32 else m.val
+    yb = ybar.double().cpu().numpy()
+    ref = O.csr_transpose_product(m.row, m.col, vals, yb)
     at, _ = m.transpose()
-    assert scaled(at, ybar.cpu().numpy(), ref, gx.cpu().numpy()) <= 1e-12
+    assert scaled(at, yb, ref, gx.double().cpu().numpy()) <= tol_adj
     occupied = op.fwd.acol.cpu().numpy().view(np.uint32) != D.PAD
     rng = np.random.default_rng(0)
-    h = 1e-6
     worst = 0.0
     with torch.no_grad():
         for _ in range(64):
-            dv = torch.tensor(rng.standard_normal(len(occupied)) * occupied, device=dev)
-            dx = torch.tensor(rng.standard_normal(n), device=dev)
+            dv = torch.tensor(rng.standard_normal(len(occupied)) * occupied, device=dev, dtype=dt)
+            dx = torch.tensor(rng.standard_normal(n), device=dev, dtype=dt)
             lp = (AG.spmv(v + h * dv, x + h * dx, op) * ybar).sum()
             lm = (AG.spmv(v - h * dv, x - h * dx, op) * ybar).sum()
             fd = ((lp - lm) / (2 * h)).item()
             an = ((gv * dv).sum() + (gx * dx).sum()).item()
             scale = ((gv * dv).abs().sum() + (gx * dx).abs().sum()).item()
             worst = max(worst, abs(fd - an) / scale)
-    assert worst < 1e-6, worst
+    assert worst < tol_fd, worst
 
 
 def test_plan_layouts_are_checked(engine):
