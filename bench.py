@@ -359,7 +359,9 @@ def main():
         log(f"[bench] AXT_NOC_H1_F64 per-launch median {med * 1e6:.2f} us, cold {cmed * 1e6:.2f} us")
 
     if not args.quick:
-        for th in (2, 4, 8, 16):
+        # the reference's sweep (tile heights 1, 4, 8, 16, 32, 48 and COM
+        # with blocks of 128/256/512, intSpmv_main.c:1668-1676) plus th = 2
+        for th in (2, 4, 8, 16, 32, 48):
             a = H.AXT(m, x, th=th)
             d = D.DeviceAXT(eng, a)
             y = eng.empty(n)
@@ -369,6 +371,14 @@ def main():
             if th == 4:
                 hosts["AXT_H4"] = a
                 h4_ev = e
+            del d, a, y
+        for bs in (128, 256, 512):
+            a = H.AXT(m, x, th=1, mode="COM", bs=bs)
+            d = D.DeviceAXT(eng, a)
+            y = eng.empty(n)
+            w, e = timed(lambda: d.int_axt(y), 20, 3, torch)
+            record(f"AXT_COM_BS{bs}_F64", M.bytes_axt(n, a.lenAX, a.lenSEC), w, e,
+                   {"beta": round(a.fd.beta, 4)})
             del d, a, y
         a4 = hosts["AXT_H4"]
         h4_bytes = M.bytes_axt(n, a4.lenAX, a4.lenSEC)

@@ -529,13 +529,30 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
 // AXT COM (tile height 1, exact CSR order, int_axt_com_h1,
 // intSpmv_main.c:1517-1566).  Slot s of the matrix is CSR entry s; sec[s] is a
 // head word (row << log | o) at the first slot of a row and at every slot
-// with s % bs == 0, and the segment it heads is slots s..s+o.  A head is
+// with s % bs == 0, and the segment it heads runs to the next head.  A head is
 // recognised by sec[s] != 0 or s % bs == 0: the second test also catches
 // row 0 with o == 0, whose word is 0 and which the reference kernel skips.
-// Each head lane labels its segment's slots with the row in LDS; each row is
-// then one lane's fused multiply-add chain over its slots in CSR order from
-// +0.0 -- the same bits as cf_CSR.
-constexpr uint32_t COM_UMAX = 256;
+// Blocks start at row starts, so a block's rows are the runs between heads
+// whose row differs from the previous head's, found for all slots at once by
+// ballots of the head flags.  Each row is then one lane's fused multiply-add
+// chain over its slots in CSR order from +0.0 -- the same bits as cf_CSR.
+#ifndef ISPV_COM_UMAX
+#define ISPV_COM_UMAX 256
+#endif
+constexpr uint32_t COM_UMAX = ISPV_COM_UMAX;
+// tiles per load step: 5 x 8 = 40 covers a block of COM_UMAX slots at any
+// alignment (at most 33 tiles), so a normal block is one step in flight
+constexpr int COM_U = (COM_UMAX + 7) / 64 + 1;
+constexpr uint32_t COM_NOHEAD = 0xFFFFFFFFu;
+#ifndef ISPV_COM_CHAIN
+#define ISPV_COM_CHAIN 8
+#endif
+constexpr int COM_CHAIN = ISPV_COM_CHAIN;
+
+template <typename AT>
+struct alignas(2 * sizeof(AT)) ComSlot {
+    AT v, x;
+};
 
 template <typename VT, typename XT, typename AT>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
@@ -543,8 +560,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
     const uint32_t *__restrict__ sec, AT *__restrict__ y, uint32_t nrows, double *partial,
     uint32_t bs, uint32_t lg)
 {
-    __shared__ AT s_v[WAVES][COM_UMAX];
-    __shared__ AT s_x[WAVES][COM_UMAX];
+    __shared__ ComSlot<AT> s_vx[WAVES][COM_UMAX];
     __shared__ uint32_t s_key[WAVES][COM_UMAX];
     __shared__ uint16_t s_seg[WAVES][COM_UMAX + 1];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -552,42 +568,42 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
     const uint32_t nu = d.uend - d.ubeg;
-    const uint32_t C = 2 * (lane & 3);
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
     AT acc = 0; // long rows: lane-strided fma partials
     // full-line mapping: 8 whole tiles per load, lane part p = l%8; values of
     // slots 2p, 2p+1 (p < 4) meet their x through a DPP row shift
     const uint32_t p = lane & 7, Cq = 2 * (p & 3);
-    (void)C;
-    for (uint32_t Tb = T0; Tb < T1; Tb += 8) {
-        const uint32_t Tr = Tb + lane / 8, u0 = 8 * Tr + Cq;
-        const uint32_t T = Tr < T1 ? Tr : T1 - 1;
-        LoadT<VT> v0, v1;
-        Pair<VT>::ld(ax + (size_t)T * 16 + 2 * p, v0, v1);
-        const AT x0 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v0), x1 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v1);
-        const bool ok0 = p < 4 && Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
-        const bool ok1 = p < 4 && Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
-        if (d.k != 0) {
-            if (ok0) acc = fma_acc<AT>((AT)v0, x0, acc);
-            if (ok1) acc = fma_acc<AT>((AT)v1, x1, acc);
-            continue;
-        }
-        const u2v hw = ntload(reinterpret_cast<const u2v *>(sec + 8 * (size_t)T + Cq));
-        if (ok0) {
-            s_v[w][u0 - d.ubeg] = (AT)v0;
-            s_x[w][u0 - d.ubeg] = x0;
-        }
-        if (ok1) {
-            s_v[w][u0 + 1 - d.ubeg] = (AT)v1;
-            s_x[w][u0 + 1 - d.ubeg] = x1;
+    for (uint32_t Tb = T0; Tb < T1; Tb += 8 * COM_U) {
+        LoadT<VT> v0[COM_U], v1[COM_U];
+        u2v hw[COM_U];
+#pragma unroll
+        for (int u = 0; u < COM_U; u++) {
+            const uint32_t Tr = Tb + 8 * u + lane / 8;
+            const uint32_t T = Tr < T1 ? Tr : T1 - 1;
+            Pair<VT>::ld(ax + (size_t)T * 16 + 2 * p, v0[u], v1[u]);
+            if (d.k == 0) hw[u] = ntload(reinterpret_cast<const u2v *>(sec + 8 * (size_t)T + Cq));
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const uint32_t u = u0 + h, word = h ? hw[1] : hw[0];
-            if ((h ? ok1 : ok0) && (word != 0 || u % bs == 0)) {
-                const uint32_t r = word >> lg, o = word & (bs - 1);
-                const uint32_t e = (u + o < d.uend ? u + o : d.uend - 1) - d.ubeg;
-                for (uint32_t i = u - d.ubeg; i <= e; i++) s_key[w][i] = r;
+        for (int u = 0; u < COM_U; u++) {
+            const uint32_t Tr = Tb + 8 * u + lane / 8, u0 = 8 * Tr + Cq;
+            const AT x0 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v0[u]);
+            const AT x1 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v1[u]);
+            const bool ok0 = p < 4 && Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
+            const bool ok1 = p < 4 && Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
+            if (d.k != 0) {
+                if (ok0) acc = fma_acc<AT>((AT)v0[u], x0, acc);
+                if (ok1) acc = fma_acc<AT>((AT)v1[u], x1, acc);
+                continue;
+            }
+            if (ok0) {
+                const bool h = hw[u][0] != 0 || u0 % bs == 0;
+                s_vx[w][u0 - d.ubeg] = {(AT)v0[u], x0};
+                s_key[w][u0 - d.ubeg] = h ? hw[u][0] >> lg : COM_NOHEAD;
+            }
+            if (ok1) {
+                const bool h = hw[u][1] != 0 || (u0 + 1) % bs == 0;
+                s_vx[w][u0 + 1 - d.ubeg] = {(AT)v1[u], x1};
+                s_key[w][u0 + 1 - d.ubeg] = h ? hw[u][1] >> lg : COM_NOHEAD;
             }
         }
     }
@@ -596,11 +612,55 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
         return;
     }
     wave_lds_sync();
-    const uint32_t nseg = find_segments(s_key[w], s_seg[w], nu);
+    // Rows: a head whose row differs from the previous head's starts a row
+    // (heads at multiples of bs inside a row repeat its row).  Slot i = 64c +
+    // lane; all reads first, then the segment starts by ballot.
+    uint32_t nseg = 0;
+    {
+        constexpr int NC = COM_UMAX / WAVE;
+        uint32_t hk[NC], prow[NC];
+        uint64_t hm[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const uint32_t i = 64 * c + lane;
+            hk[c] = i < nu ? s_key[w][i] : COM_NOHEAD;
+            hm[c] = __ballot(hk[c] != COM_NOHEAD);
+        }
+        int32_t last = -1; // last head in the chunks before c (wave-uniform)
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const uint64_t m = hm[c] & lanemask_lt();
+            const int32_t pos = m ? 64 * c + 63 - __builtin_clzll(m) : last;
+            prow[c] = pos >= 0 ? s_key[w][pos] : COM_NOHEAD;
+            if (hm[c]) last = 64 * c + 63 - __builtin_clzll(hm[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const bool start = hk[c] != COM_NOHEAD && hk[c] != prow[c];
+            const uint64_t sm = __ballot(start);
+            if (start) s_seg[w][nseg + __popcll(sm & lanemask_lt())] = (uint16_t)(64 * c + lane);
+            nseg += __popcll(sm);
+        }
+        if (lane == 0) s_seg[w][nseg] = (uint16_t)nu;
+    }
+    wave_lds_sync();
     for (uint32_t s = lane; s < nseg; s += WAVE) {
         AT a = 0;
-        for (uint32_t i = s_seg[w][s], b = s_seg[w][s + 1]; i < b; i++)
-            a = fma_acc<AT>(s_v[w][i], s_x[w][i], a);
+        const uint32_t b = s_seg[w][s + 1];
+        uint32_t i = s_seg[w][s];
+        // COM_CHAIN slots read before their fmas, so LDS latency is paid once
+        // per COM_CHAIN
+        for (; i + COM_CHAIN <= b; i += COM_CHAIN) {
+            ComSlot<AT> e[COM_CHAIN];
+#pragma unroll
+            for (int q = 0; q < COM_CHAIN; q++) e[q] = s_vx[w][i + q];
+#pragma unroll
+            for (int q = 0; q < COM_CHAIN; q++) a = fma_acc<AT>(e[q].v, e[q].x, a);
+        }
+        for (; i < b; i++) {
+            const ComSlot<AT> e = s_vx[w][i];
+            a = fma_acc<AT>(e.v, e.x, a);
+        }
         const uint32_t r = s_key[w][s_seg[w][s]];
         if (r < nrows) y[r] = a;
     }
@@ -976,7 +1036,7 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
             }
         }
         uptr[D.nrows] = D.nnz;
-        int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, COM_UMAX, AXT_RMAX, 128,
+        int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, COM_UMAX, AXT_RMAX, COM_UMAX / 2,
                                  LONG_SLOTS, plan);
         free(uptr);
         if (st) return st;

@@ -177,7 +177,7 @@ def test_k1(engine, lap256, mats, tiny10):
 # ---------------------------------------------------------------- AXT
 
 
-@pytest.mark.parametrize("th", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("th", [1, 2, 4, 8, 16, 32, 48])
 def test_axt_lap256(engine, lap256, th):
     m, x = lap256
     a = H.AXT(m, x, th=th)
@@ -189,7 +189,7 @@ def test_axt_lap256(engine, lap256, th):
     assert scaled(m, x, O.cf_csr(m.row, m.col, m.val, x), g) <= TOL64
 
 
-@pytest.mark.parametrize("th", [1, 2, 4, 8, 16, 48])
+@pytest.mark.parametrize("th", [1, 2, 4, 8, 16, 32, 48])
 def test_axt_mats(engine, mats, tiny10, th):
     fx, tm, tx = tiny10
     cases = [("tiny10", tm, tx, O.cf_csr(tm.row, tm.col, tm.val, tx))] + list(mats)
