@@ -678,6 +678,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
     const double *__restrict__ ax, const uint32_t *__restrict__ sec, double *__restrict__ y,
     uint32_t nrows, double *partial, uint32_t th_rt)
 {
+    static_assert(TH <= 16 || TH % 8 == 0, "tall tiles load brick-rows in chunks of eight");
     // th = 2 blocks hold 256 columns (8 KB, like th = 4's 128): capacity per height
     constexpr uint32_t UM = TH == 2 ? TH2_UMAX : AXT_UMAX;
     __shared__ double s_val[WAVES][UM];
@@ -697,7 +698,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
     constexpr int G = TH == 2 ? (int)(TH2_UMAX / 64) : (TH == 0 || TH >= 8) ? 1 : 2;
     for (uint32_t Tb = T0; Tb < T1; Tb += 8 * G) {
         double a0[G], a1[G];
-        if constexpr (TH > 0) {
+        if constexpr (TH > 0 && TH <= 16) {
             d2v v[G][TH];
 #pragma unroll
             for (int g = 0; g < G; g++) {
@@ -719,15 +720,25 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
                 }
             }
         } else {
+            // tall tiles: brick-rows in chunks of eight loads issued together
+            // (the fma chain still runs over the brick-rows in order)
             const uint32_t Tr = Tb + lane / 8;
             const size_t T = Tr < T1 ? Tr : T1 - 1;
             a0[0] = 0.0;
             a1[0] = 0.0;
-            for (uint32_t R = 0; R < th; R++) {
-                d2v v = ntload(reinterpret_cast<const d2v *>(ax + (T * th + R) * 16 + 2 * p));
-                double x0 = dpp_f64<DPP_ROW_SHL4>(v[0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[1]);
-                a0[0] = fma(v[0], x0, a0[0]);
-                a1[0] = fma(v[1], x1, a1[0]);
+            for (uint32_t R0 = 0; R0 < th; R0 += 8) {
+                d2v v[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    if (TH > 0 || R0 + r < th) // whole chunks when TH (a multiple of 8) is known
+                        v[r] = ntload(reinterpret_cast<const d2v *>(ax + (T * th + R0 + r) * 16 + 2 * p));
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    if (TH == 0 && R0 + r >= th) break;
+                    const double x0 = dpp_f64<DPP_ROW_SHL4>(v[r][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[r][1]);
+                    a0[0] = fma(v[r][0], x0, a0[0]);
+                    a1[0] = fma(v[r][1], x1, a1[0]);
+                }
             }
         }
 #pragma unroll
@@ -979,6 +990,8 @@ static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, con
         case 4: axt_noc_fl_kernel<4><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;
         case 8: axt_noc_fl_kernel<8><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;
         case 16: axt_noc_fl_kernel<16><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;
+        case 32: axt_noc_fl_kernel<32><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;
+        case 48: axt_noc_fl_kernel<48><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th); break;
         default: axt_noc_fl_kernel<0><<<grid, blk, 0, s>>>(p->d_blocks, nl, nb, a, sec, yy, p->nrows, p->d_partial, th);
         }
     } else {
