@@ -437,6 +437,14 @@ def main():
         fwd32 = e
         record("AXT_GATHER_H1_F32_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, sv=4, sx=4, sy=4),
                w, e)
+        k32 = 32  # 16 right-hand sides per pass
+        X32 = torch.rand(n * k32, dtype=torch.float32, device=eng.torch_device)
+        Y32 = torch.empty(n * k32, dtype=torch.float32, device=eng.torch_device)
+        w, e = timed(lambda: g32.matvec(X32, Y32, k=k32), 10, 2, torch)
+        record("AXT_GATHER_H1_F32_K32",
+               M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, k=k32, sv=4, sx=4, sy=4), w, e,
+               {"GFLOPs": round(M.flops(nnz, k32) / e / 1e9, 1)})
+        del X32, Y32
         adj32 = D.AxtAdjoint(eng, m, g32)
         yb32 = yb.float()
         xb32 = torch.empty(n, dtype=torch.float32, device=eng.torch_device)

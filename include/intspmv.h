@@ -157,8 +157,9 @@ int spmv_axt(intspmv_ctx *ctx, const intspmv_plan *plan, const void *ax, const v
 int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval,
                     const uint32_t *acol, const uint32_t *sec, const void *x, uint32_t ldx,
                     void *y, uint32_t ldy, uint32_t k, uint32_t kc);
-/* kc: right-hand sides per pass over the matrix (1, 2, 4 or 8); 0 = the largest
- * power of two <= min(k, 8) for each pass. */
+/* kc: right-hand sides per pass over the matrix (1, 2, 4 or 8, and 16 for F32
+ * and BF16 values); 0 = for each pass the largest of these <= what is left
+ * of k (16 only for F32/BF16). */
 /* Value gradient on the pattern: vbar[s] = ybar[row(s)] * x[acol[s]] (0 on
  * padding), one entry per slot (lenAX/2).  F64 or F32. */
 int spmv_axt_grad_vals(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_t *acol,

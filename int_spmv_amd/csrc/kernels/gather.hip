@@ -137,7 +137,9 @@ static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t
     const bool xb_ok = xbuf_fits(p->nrows, ldx, sizeof(XT));
     for (uint32_t j0 = 0; j0 < k;) {
         const uint32_t rem = k - j0;
-        const uint32_t kc = kcf ? kcf : (rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1)));
+        const uint32_t kc = kcf ? kcf
+                                : (rem >= 16 && sizeof(AT) == 4) ? 16
+                                : (rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1)));
         const uint32_t kv = rem < kc ? rem : kc;
         const bool xb = xb_ok && kc < 16 / sizeof(XT);
 #define ISPV_GX(KC, H1, XB)                                                                     \
@@ -163,6 +165,9 @@ static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t
         ISPV_G(2)
         ISPV_G(4)
         ISPV_G(8)
+        if constexpr (sizeof(AT) == 4) {
+            ISPV_G(16)
+        }
 #undef ISPV_G
 #undef ISPV_GX
         int st = ispv_launch_check(p->ctx);
@@ -189,7 +194,9 @@ extern "C" int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const
     // the column-map blocks stage at most G_UMAX units: a plan built for the
     // reference-layout kernels with larger blocks is refused
     if (!gather_plan_ok(ctx, plan) || plan->umax > G_UMAX || k == 0 || k > PARTIAL_STRIDE ||
-        ldx < k || ldy < k || (kc != 0 && kc != 1 && kc != 2 && kc != 4 && kc != 8))
+        ldx < k || ldy < k ||
+        (kc != 0 && kc != 1 && kc != 2 && kc != 4 && kc != 8 &&
+         !(kc == 16 && plan->axt.val_type != INTSPMV_F64)))
         return INTSPMV_ERR_ARG;
     if (plan->nrows && (!x || !y || (plan->nunits && (!aval || !acol || !sec))))
         return INTSPMV_ERR_ARG;

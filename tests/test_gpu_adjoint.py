@@ -102,6 +102,39 @@ def test_gather_wide_x_plain_loads(engine, mats, vt):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("vt", [D.F32, D.BF16])
+def test_gather_sixteen_per_pass(engine, mats, vt):
+    """FP32 sums take 16 right-hand sides per pass (the default for k >= 16):
+    each right-hand side keeps its own sums in the same order, so the bits
+    equal passes of 8; FP64 refuses 16."""
+    import torch
+    for name, m in mats[:2]:
+        n = m.nrows
+        for th, k in ((1, 16), (1, 21), (4, 32)):
+            X = np.stack([H.uniform(n, seed=80 + j) for j in range(k)], axis=1).astype(np.float32)
+            a = H.AXT(m, np.zeros(n), th=th, want_acol=True)
+            g = D.DeviceAXTGather(engine, a, val_type=vt, x_type=D.F32)
+            Xd = engine.upload(X.reshape(-1))
+            Y16, Y8 = _nan(engine, n * k, np.float32), _nan(engine, n * k, np.float32)
+            g.matvec(Xd, Y16, k=k)
+            g.matvec(Xd, Y8, k=k, kc=8)
+            torch.cuda.synchronize()
+            assert bitwise_equal(Y16.cpu().numpy(), Y8.cpu().numpy()), (name, th, k)
+            Yg = _host(Y16, n * k).reshape(n, k)
+            for j in (0, k - 1):
+                yr = O.cf_csr(m.row, m.col, m.val, X[:, j].astype(np.float64))
+                tol = 1e-6 if vt == D.F32 else 1e-2
+                assert scaled(m, X[:, j].astype(np.float64), yr, Yg[:, j]) <= tol, (name, th, k, j)
+    m = mats[0][1]
+    a = H.AXT(m, np.zeros(m.nrows), th=1, want_acol=True)
+    g = D.DeviceAXTGather(engine, a)
+    Y = _nan(engine, m.nrows * 16)
+    st = D.lib().spmv_axt_gather(engine.h, g.plan.h, D._dptr(g.aval), D._dptr(g.acol),
+                                 D._dptr(g.sec), D._dptr(engine.upload(np.zeros(m.nrows * 16))),
+                                 16, D._dptr(Y), 16, 16, 16)
+    assert st == -1
+
+
 @pytest.mark.parametrize("vt,xt,tol", [(D.F32, D.F32, 1e-6), (D.BF16, D.F32, 1e-2)])
 def test_gather_precision(engine, mats, vt, xt, tol):
     for name, m in mats:
