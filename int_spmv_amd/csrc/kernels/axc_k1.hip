@@ -96,7 +96,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
     for (uint32_t r = r0; r < d.rend; r += WAVE) {
         uint32_t a = (r == r0 ? ra : brp[r]) / 16 - d.ubeg, b = (r == r0 ? rb : brp[r + 1]) / 16 - d.ubeg;
         double sum = 0.0;
-        for (uint32_t i = a; i < b; i++) sum = sum + s_val[w][i];
+        in_order_chunks<8, double>(a, b, [&](uint32_t i) { return s_val[w][i]; },
+                                   [&](double v) { sum = sum + v; });
         y[r] = sum;
     }
 }

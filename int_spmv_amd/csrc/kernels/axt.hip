@@ -649,7 +649,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
         const uint32_t b = s_seg[w][s + 1];
         uint32_t i = s_seg[w][s];
         // COM_CHAIN slots read before their fmas, so LDS latency is paid once
-        // per COM_CHAIN
+        // per COM_CHAIN (a masked last chunk measured slower here)
         for (; i + COM_CHAIN <= b; i += COM_CHAIN) {
             ComSlot<AT> e[COM_CHAIN];
 #pragma unroll

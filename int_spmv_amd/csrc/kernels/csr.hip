@@ -157,7 +157,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
     for (uint32_t r = r0; r < d.rend; r += WAVE) {
         uint32_t a = (r == r0 ? ra : row[r]) - d.ubeg, b = (r == r0 ? rb : row[r + 1]) - d.ubeg;
         double acc = 0.0;
-        for (uint32_t i = a; i < b; i++) acc = fma(s_v[w][i], s_x[w][i], acc);
+        in_order_chunks<8, d2v>(a, b, [&](uint32_t i) { return d2v{s_v[w][i], s_x[w][i]}; },
+                                [&](d2v e) { acc = fma(e[0], e[1], acc); });
         y[r] = acc;
     }
 }

@@ -269,9 +269,27 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
         AT acc[KC];
 #pragma unroll
         for (int j = 0; j < KC; j++) acc[j] = 0;
-        for (uint32_t i = a; i < b; i++)
+        if constexpr (KC <= 4) {
+            struct U {
+                AT v[KC];
+            };
+            in_order_chunks<8 / KC, U>(
+                a, b,
+                [&](uint32_t i) {
+                    U u;
 #pragma unroll
-            for (int j = 0; j < KC; j++) acc[j] = acc[j] + sv[i * KC + j];
+                    for (int j = 0; j < KC; j++) u.v[j] = sv[i * KC + j];
+                    return u;
+                },
+                [&](const U &u) {
+#pragma unroll
+                    for (int j = 0; j < KC; j++) acc[j] = acc[j] + u.v[j];
+                });
+        } else {
+            for (uint32_t i = a; i < b; i++)
+#pragma unroll
+                for (int j = 0; j < KC; j++) acc[j] = acc[j] + sv[i * KC + j];
+        }
         if (r < nrows)
 #pragma unroll
             for (int j = 0; j < KC; j++)

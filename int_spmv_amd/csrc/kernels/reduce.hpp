@@ -65,6 +65,25 @@ struct LaneSum<float> {
     __device__ __forceinline__ float value() const { return k.value(); }
 };
 
+// Calls use(get(i)) for i = a .. b-1 in order, with the reads of CH indexes
+// issued together and the last b-a mod CH one by one, so a row's chain
+// waits for LDS about once per CH entries instead of once per entry.  The
+// order of use() is unchanged.  (Reading a whole last chunk, masked, measured
+// slower.)
+template <int CH, typename T, typename Get, typename Use>
+__device__ __forceinline__ void in_order_chunks(uint32_t a, uint32_t b, Get get, Use use)
+{
+    uint32_t i = a;
+    for (; i + CH <= b; i += CH) {
+        T t[CH];
+#pragma unroll
+        for (int q = 0; q < CH; q++) t[q] = get(i + q);
+#pragma unroll
+        for (int q = 0; q < CH; q++) use(t[q]);
+    }
+    for (; i < b; i++) use(get(i));
+}
+
 // Finds the runs of equal keys (rows) of a normal block: sseg[0..nseg) are
 // the first units of the runs, sseg[nseg] = nu.  Returns nseg.
 __device__ __forceinline__ uint32_t find_segments(const uint32_t *sk, uint16_t *sseg, uint32_t nu)
@@ -132,7 +151,8 @@ __device__ __forceinline__ void reduce_keyed(const AT *sv, const uint32_t *sk, u
     for (uint32_t s = lane; s < nseg; s += WAVE) {
         uint32_t a = sseg[s], b = sseg[s + 1];
         AT acc = 0;
-        for (uint32_t i = a; i < b; i++) acc = acc + sv[i];
+        in_order_chunks<8, AT>(a, b, [&](uint32_t i) { return sv[i]; },
+                               [&](AT v) { acc = acc + v; });
         uint32_t r = sk[a];
         if (r < nrows) y[r] = acc;
     }
