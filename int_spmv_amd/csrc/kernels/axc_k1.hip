@@ -277,6 +277,8 @@ extern "C" int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkN
     p->nrows = nrows;
     p->chunkNum = chunkNum;
     p->nblocks = (uint32_t)items.size();
+    p->nunits = 0; // the longest chunk's steps: 0 means val/col are never read
+    for (uint32_t c = 0; c < chunkNum; c++) p->nunits = nmc_host[c] > p->nunits ? nmc_host[c] : p->nunits;
     p->nlong_wg = nwide; // number of wide (one-item) waves
     p->nsplit_rows = (uint32_t)splits.size();
     int st = ispv_plan_upload(ctx, p, items.data(), sizeof(K1Item), items.size(), 8,
@@ -298,7 +300,8 @@ extern "C" int spmv_k1(intspmv_ctx *ctx, const intspmv_plan *plan, const double 
     int st = ispv_enter(ctx);
     if (st) return st;
     if (plan->nblocks == 0) return INTSPMV_OK;
-    if (!chp || !x || !y || !val || !col) return INTSPMV_ERR_ARG;
+    // a matrix without entries (every chunk 0 steps) reads no val/col
+    if (!chp || !x || !y || (plan->nunits && (!val || !col))) return INTSPMV_ERR_ARG;
     const uint32_t nwide = plan->nlong_wg;
     uint32_t waves = nwide + (plan->nblocks - nwide + 7) / 8;
     dim3 grid((waves + WAVES - 1) / WAVES);

@@ -1,0 +1,107 @@
+"""GPU: degenerate matrices through every entry point -- a 1x1 matrix, a
+matrix whose rows are all empty, a single empty row, and a matrix whose only
+entries sit in its last row.  y is pre-filled with NaN, so a row that no
+kernel writes fails; the expected values come from the oracle (cf_CSR), and
+with at most two products per row every format must give its bits."""
+import numpy as np
+import pytest
+
+import oracle as O
+from int_spmv_amd import device as D
+from int_spmv_amd import host as H
+
+from .util import bitwise_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    u32 = np.uint32
+    return [
+        ("one_by_one", H.CSR.from_arrays(np.array([0, 1], u32), np.array([0], u32),
+                                         np.array([2.5]))),
+        ("all_empty5", H.CSR.from_arrays(np.zeros(6, u32), np.zeros(0, u32), np.zeros(0))),
+        ("one_empty", H.CSR.from_arrays(np.zeros(2, u32), np.zeros(0, u32), np.zeros(0))),
+        ("last_only", H.CSR.from_arrays(np.array([0, 0, 0, 0, 2], u32), np.array([0, 3], u32),
+                                        np.array([1.0, -2.0]))),
+    ]
+
+
+def _nan(eng, n, dtype=np.float64):
+    t = eng.empty(max(n, 1), dtype)
+    t.fill_(float("nan"))
+    return t
+
+
+def _host(t, n):
+    import torch
+    torch.cuda.synchronize()
+    return t.reshape(-1)[:n].double().cpu().numpy()
+
+
+@pytest.mark.parametrize("name,m", _cases(), ids=[c[0] for c in _cases()])
+def test_every_format_on_degenerate_matrices(engine, name, m):
+    n = m.nrows
+    x = H.uniform(n, seed=4)
+    yr = O.cf_csr(m.row, m.col, m.val, x)
+    xd = engine.upload(x)
+    got = {}
+    dc = D.DeviceCSR(engine, m)
+    y = _nan(engine, n)
+    dc.spmv(xd, y)
+    got["csr"] = _host(y, n)
+    y = _nan(engine, n)
+    dc.cf_csr(xd, y)
+    got["csr_ordered"] = _host(y, n)
+    y = _nan(engine, n)
+    D.DeviceAXC(engine, H.AXC(m, x)).int_axc(y)
+    got["axc"] = _host(y, n)
+    k = H.K1(m)
+    dk = D.DeviceK1(engine, k)
+    y = _nan(engine, n)
+    dk.int_k1p(xd, y)
+    got["k1p"] = _host(y, n)
+    res = _nan(engine, k.chunkNum * 8)
+    dk.int_k1(xd, res)
+    y = _nan(engine, n)
+    dk.order(res, y)
+    got["k1"] = _host(y, n)
+    for th in (1, 2, 4, 8, 32):
+        y = _nan(engine, n)
+        D.DeviceAXT(engine, H.AXT(m, x, th=th)).int_axt(y)
+        got[f"axt_th{th}"] = _host(y, n)
+    for bs in (2, 128):
+        y = _nan(engine, n)
+        D.DeviceAXT(engine, H.AXT(m, x, th=1, mode="COM", bs=bs)).int_axt(y)
+        got[f"com_bs{bs}"] = _host(y, n)
+    ag = H.AXT(m, np.zeros(n), th=1, want_acol=True)
+    g = D.DeviceAXTGather(engine, ag)
+    y = _nan(engine, n)
+    g.matvec(xd, y)
+    got["colmap"] = _host(y, n)
+    for name_f, got_y in got.items():
+        assert bitwise_equal(got_y, yr), (name, name_f, got_y, yr)
+    # multi-RHS, adjoint and a batch of the matrix
+    k3 = 3
+    X = np.stack([H.uniform(n, seed=10 + j) for j in range(k3)], axis=1)
+    Y = _nan(engine, n * k3)
+    g.matvec(engine.upload(X.reshape(-1)), Y, k=k3)
+    Yg = _host(Y, n * k3).reshape(n, k3)
+    for j in range(k3):
+        assert bitwise_equal(Yg[:, j], O.cf_csr(m.row, m.col, m.val, X[:, j].copy())), (name, j)
+    adj = D.AxtAdjoint(engine, m, g)
+    ybar = H.uniform(n, seed=12)
+    xb = _nan(engine, n)
+    adj.adjoint(engine.upload(ybar), xb)
+    ref = O.csr_transpose_product(m.row, m.col, m.val, ybar)
+    assert bitwise_equal(_host(xb, n), ref), name
+    import torch
+    Yb = torch.full((n * k3,), float("nan"), dtype=torch.float32, device=engine.torch_device)
+    Xb = torch.tensor(X.astype(np.float32).reshape(-1), device=engine.torch_device)
+    b = D.DeviceBatch(engine, [dict(a=ag, X=Xb, Y=Yb, k=k3)], D.F32, D.F32)
+    b.launch()
+    Ybh = _host(Yb, n * k3).reshape(n, k3)
+    v32 = m.val.astype(np.float32).astype(np.float64)
+    for j in range(k3):
+        want = O.cf_csr(m.row, m.col, v32, X[:, j].astype(np.float32).astype(np.float64))
+        assert np.allclose(Ybh[:, j], want, rtol=1e-6, atol=0), (name, j)
