@@ -48,6 +48,9 @@ __host__ __device__ constexpr uint32_t red_umax(int th)
 {
     return (th == 2 || th == 4) ? ISPV_RED_TH_UMAX : ISPV_AXT_UMAX;
 }
+#ifndef ISPV_TALL_UMAX
+#define ISPV_TALL_UMAX 32 // th >= 16, FP64 reference layout, dense rows (32 columns of 2+ KB)
+#endif
 #ifndef ISPV_TH8_UMAX
 #define ISPV_TH8_UMAX 64 // th = 8, FP64 and FP32 reference layouts
 #endif
@@ -1098,6 +1101,7 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
     const uint32_t umax = small ? ISPV_H1_F64_UMAX : th == 1 ? ISPV_H1_RED_UMAX
                         : (th == 2 && fl64) ? TH2_UMAX
                         : (th == 8 && dense_rows) ? ISPV_TH8_UMAX
+                        : (th >= 16 && fl64 && dense_rows) ? ISPV_TALL_UMAX
                         : (fl32 || planar32) ? red_umax(th) : AXT_UMAX;
     uint32_t long_units = th == 1 ? (small ? ISPV_H1_F64_LONG : ISPV_H1_LONG)
                                   : (512 / th < 4 ? 4 : (512 / th > umax / 2 ? umax / 2 : 512 / th));

@@ -203,8 +203,9 @@ def test_axt_mats(engine, mats, tiny10, th):
         # rows that fit one wave block have the oracle's bits
         per = 8 if th == 1 else th
         units = (np.diff(m.row.astype(np.int64)) + per - 1) // per
-        # long-row thresholds of the plans (th = 8: 64-column blocks on dense rows)
-        lim = 48 if th == 1 else max(4, min(32 if th == 8 else 64, 512 // th))
+        # long-row thresholds of the plans (th = 8: 64-column blocks on dense
+        # rows; th >= 16: 32-column blocks on dense rows)
+        lim = 48 if th == 1 else max(4, min(32 if th == 8 else 16 if th >= 16 else 64, 512 // th))
         ok = units <= lim
         assert bitwise_equal(g[ok], O.int_axt(a)[ok]), (name, th)
 
