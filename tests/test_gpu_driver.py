@@ -17,14 +17,19 @@ KERNELS = ["spmv_csr_ordered", "spmv_csr", "spmv_axc", "spmv_k1", "spmv_k1p"] + 
 
 
 def _table(out: str) -> dict:
+    """The reference's kernels' table (seven columns) and the scaled errors."""
     lines = out.splitlines()
     i = next(k for k, line in enumerate(lines) if line.startswith("kernels' results"))
+    j = next(k for k, line in enumerate(lines) if line.startswith("kernels' scaled errors"))
     rows = {}
-    for line in lines[i + 3:]:
+    for line in lines[i + 3:j - 1]:
         f = line.split()
-        if len(f) != 8:
-            continue
-        rows[f[0]] = dict(et=float(f[1]), aerr=float(f[4]), serr=float(f[7]))
+        assert len(f) == 7, line
+        rows[f[0]] = dict(et=float(f[1]), aerr=float(f[4]))
+    for line in lines[j + 3:]:
+        f = line.split()
+        if len(f) == 2 and f[0] in rows:
+            rows[f[0]]["serr"] = float(f[1])
     return rows
 
 
