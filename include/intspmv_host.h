@@ -115,9 +115,11 @@ int intspmv_csr_from_coo(uint32_t nrows, size_t n_entries, const uint32_t *ri,
                          const uint32_t *ci, const double *v, intspmv_csr *out);
 
 /* ---- statistics (intSpmv_main.c:253-302) ------------------------------- */
-/* Fills rl[] and rmin/rave/rmax/rsd/bw.  When sta_path is not NULL the
- * statistics are also written there in the reference's .sta layout. */
-int intspmv_csr_stats(intspmv_csr *m, const char *sta_path);
+/* Fills rl[] and rmin/rave/rmax/rsd/bw.  When mat_file_name is not NULL the
+ * statistics are also written, in the reference's layout, to the file of that
+ * name with its extension (from the first '.' of the last path component)
+ * replaced by .sta -- printMatrixStats(matFileName, ...). */
+int intspmv_csr_stats(intspmv_csr *m, const char *mat_file_name);
 /* Format data of plain CSR (intSpmv_main.c:324-357). */
 void intspmv_csr_format_data(const intspmv_csr *m, intspmv_format_data *fd);
 

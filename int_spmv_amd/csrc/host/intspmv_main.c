@@ -199,9 +199,7 @@ int main(int argc, char **argv)
         fprintf(stderr, "cannot read / generate matrix (%d)\n", st);
         return EXIT_FAILURE;
     }
-    char sta[512];
-    snprintf(sta, sizeof sta, "%s.sta", path ? path : "generated");
-    if (intspmv_csr_stats(&m, path ? sta : NULL)) return EXIT_FAILURE;
+    if (intspmv_csr_stats(&m, path)) return EXIT_FAILURE; /* <matrix>.sta beside a read matrix */
     CK(intspmv_create(&ctx, device));
 
     const uint32_t n = m.nrows;

@@ -14,7 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-int intspmv_csr_stats(intspmv_csr *m, const char *sta_path)
+int intspmv_csr_stats(intspmv_csr *m, const char *mat_file_name)
 {
     if (!m || !m->row || !m->rl) return INTSPMV_HOST_EARG;
     uint32_t rmin = 1000000000u, rmax = 0, bw = 0;
@@ -40,13 +40,24 @@ int intspmv_csr_stats(intspmv_csr *m, const char *sta_path)
     m->rmax = rmax;
     m->rsd = sqrt(var > 0.0 ? var : 0.0);
     m->bw = bw;
-    if (sta_path) {
-        FILE *fh = fopen(sta_path, "w+");
+    if (mat_file_name) {
+        /* <name>.sta beside the matrix: the extension of the file name (from
+         * its first '.') becomes .sta, as the reference's strtok does for a
+         * path without dots in its directories */
+        size_t len = strlen(mat_file_name);
+        char *sta = (char *)malloc(len + 5);
+        if (!sta) return INTSPMV_HOST_ENOMEM;
+        memcpy(sta, mat_file_name, len + 1);
+        char *base = strrchr(sta, '/');
+        char *dot = strchr(base ? base + 1 : sta, '.');
+        strcpy(dot ? dot : sta + len, ".sta");
+        FILE *fh = fopen(sta, "w+");
+        free(sta);
         if (!fh) return INTSPMV_HOST_EIO;
         fprintf(fh, "------------------------------------\n");
         fprintf(fh, "matrix's statistics\n");
         fprintf(fh, "------------------------------------\n");
-        fprintf(fh, "name:  %28s\n", sta_path);
+        fprintf(fh, "name:  %28s\n", mat_file_name);
         fprintf(fh, "nrows: %28u\n", m->nrows);
         fprintf(fh, "nnz:   %28u\n", m->nnz);
         fprintf(fh, "rmin:  %28u\n", m->rmin);

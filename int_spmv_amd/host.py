@@ -250,9 +250,12 @@ class CSR:
     def rl(self) -> np.ndarray:
         return _view(self._s.rl, self.nrows, np.uint32)
 
-    def stats(self, sta_path: str | None = None) -> dict:
+    def stats(self, mat_file_name: str | None = None) -> dict:
+        """Row statistics (printMatrixStats, intSpmv_main.c:253-302); with a
+        matrix file name also writes <name>.sta beside it."""
         _check(lib().intspmv_csr_stats(C.byref(self._s),
-                                       os.fsencode(sta_path) if sta_path else None), "stats")
+                                       os.fsencode(mat_file_name) if mat_file_name else None),
+               "stats")
         s = self._s
         return dict(nrows=s.nrows, nnz=s.nnz, rmin=s.rmin, rave=s.rave, rmax=s.rmax,
                     rsd=s.rsd, bw=s.bw)

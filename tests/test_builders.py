@@ -1,7 +1,5 @@
 """Host builders (C) against closed forms, the hand-derived fixture and the
 independent numpy restatement; I/O round trips; error paths."""
-import os
-
 import numpy as np
 import pytest
 
@@ -160,12 +158,27 @@ def test_bad_files(tmp_path):
 
 
 def test_sta_file(tmp_path):
+    """printMatrixStats (intSpmv_main.c:253-302): <name>.sta beside the matrix
+    file, its name line the matrix file name.  The 4x4-grid Laplacian has 4
+    rows of 3, 8 of 4 and 4 of 5 entries: mean 4, variance 0.5."""
     m = H.CSR.laplace2d(4)
-    p = str(tmp_path / "lap.sta")
-    s = m.stats(p)
+    mat = str(tmp_path / "lap.mtx")
+    s = m.stats(mat)
     assert (s["rmin"], s["rmax"], s["bw"]) == (3, 5, 4)
-    txt = open(p).read()
-    assert "nnz:" in txt and "rave:" in txt and os.path.getsize(p) > 100
+    rsd = 0.5 ** 0.5
+    want = ("------------------------------------\n"
+            "matrix's statistics\n"
+            "------------------------------------\n"
+            f"name:  {mat:>28s}\n"
+            f"nrows: {16:28d}\n"
+            f"nnz:   {64:28d}\n"
+            f"rmin:  {3:28d}\n"
+            f"rave:  {4.0:28.2f}\n"
+            f"rmax:  {5:28d}\n"
+            f"rsd:   {rsd:28.2f}\n"
+            f"rsdp:  {rsd / 4 * 100:28.2f}\n"
+            f"bw:    {4:28d}\n")
+    assert open(str(tmp_path / "lap.sta")).read() == want
 
 
 def test_generators_deterministic():
