@@ -192,7 +192,10 @@ def cpu_leg(m, x, a1, hosts, formats, y1, head_bytes, workload, quick):
 
     def serr(y):
         return O.scaled_err(m.row, m.col, m.val, x, yref, y)[0]
-    formats["AXT_NOC_H1_F64"]["max_scaled_err_vs_csr"] = serr(y1[:n].cpu().numpy())
+    yh = y1[:n].cpu().numpy()
+    formats["AXT_NOC_H1_F64"]["max_scaled_err_vs_csr"] = serr(yh)
+    a_err, r_err, pos = O.get_errors(yref, yh)  # the reference's getErrors (404-425)
+    formats["AXT_NOC_H1_F64"]["getErrors"] = {"aErr": a_err, "rErr": r_err, "rowInd": pos}
     for nm in ("AXT_NOC_H1_F32", "AXT_NOC_H1_BF16", "AXT_NOC_H4_F32", "AXT_NOC_H4_BF16"):
         if nm in formats and "_y" in formats[nm]:
             yy = formats[nm].pop("_y")[:n].double().cpu().numpy()
