@@ -88,7 +88,9 @@ int spmv_axc(intspmv_ctx *ctx, const intspmv_plan *plan, const double *ax, const
  * Replaces int_k1 + orderArrayK1 (intSpmv_main.c:943-976) and int_k1p
  * (intSpmv_main.c:1026-1066).  With permi == NULL the permuted result
  * res[chunkNum*8] is written (int_k1); otherwise y[permi[i]] (int_k1p, which
- * masks the padded lanes of the last chunk). */
+ * masks the padded lanes of the last chunk).  val must be 16-byte and col
+ * 8-byte aligned (as device allocations are; the reference's _mm_malloc gives
+ * 64). */
 int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkNum, const uint32_t *nmc_host,
                     intspmv_plan **plan);
 int spmv_k1(intspmv_ctx *ctx, const intspmv_plan *plan, const double *val, const uint32_t *col,

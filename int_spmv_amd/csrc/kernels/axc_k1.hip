@@ -9,7 +9,11 @@
 //   length-sorted order, element k of lane l at chp[c] + 8k + l.  Eight lanes
 //   per chunk, one fma chain per lane in k order (the reference's lane-wise
 //   vtSum).  Chunks with more than K1_WIDE steps get a whole wave each (and
-//   are cut into K1_PIECE-step pieces that meet in the plan scratch).  The result goes to res[8c + l] (int_k1) or is
+//   are cut into K1_PIECE-step pieces that meet in the plan scratch); the
+//   others are packed into blocks of consecutive chunks (at most K1_BLK
+//   slots, contiguous in val/col): a wave streams a block with 16-byte pair
+//   loads, gathers x and stages both in LDS, then each lane runs one row's
+//   chain from LDS.  The result goes to res[8c + l] (int_k1) or is
 //   scattered to y[permi[8c + l]] for the valid lanes (int_k1p).
 #include <stdlib.h>
 
@@ -26,8 +30,14 @@ constexpr uint32_t AXC_UMAX = 128; // LDS capacity (bricks per block)
 constexpr uint32_t AXC_UMAX_DENSE = 64, AXC_LONG_DENSE = 48;
 constexpr uint32_t AXC_RMAX = 256;
 constexpr uint32_t AXC_LONG = 64;
-constexpr uint32_t K1_WIDE = 32;   // chunks with more steps get a whole wave
+#ifndef ISPV_K1_WIDE
+#define ISPV_K1_WIDE 64
+#endif
+constexpr uint32_t K1_WIDE = ISPV_K1_WIDE; // chunks with more steps get a whole wave
 constexpr uint32_t K1_PIECE = 512; // steps per wide item (longer chunks split)
+constexpr uint32_t K1_BLK = 512;   // slots per staged block (a multiple of 128)
+constexpr uint32_t K1_BLK_CHUNKS = 64; // chunks per staged block
+static_assert(K1_BLK % 128 == 0 && 8 * K1_WIDE <= K1_BLK, "a narrow chunk fits a staged block");
 
 __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
@@ -102,33 +112,36 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
     }
 }
 
-// K1 work item: chunk c, steps [kbeg, kend); k > 1 marks pieces b0..b0+k-1.
+// K1 work item, wide: chunk c, steps [kbeg, kend); k > 1 marks pieces
+// b0..b0+k-1.  Staged block (after the wide items): chunks [c, kbeg), slots
+// [kend, b0) of val/col.
 struct K1Item {
     uint32_t c, kbeg, kend, b0, k, pad0, pad1, pad2;
 };
 
-// Two sections of waves.  The first nwide waves take one item each with all
-// 64 lanes: lane (l%8 = row of the chunk, l/8 = step slice) runs the steps
-// k = kbeg + l/8 (mod 8), so every load instruction reads 64 consecutive
-// entries, and the eight slices are added by a tree at the end (long chunks,
-// where eight lanes alone would serialise thousands of dependent gathers).
-// The remaining waves take eight short items each, eight lanes per item, one
-// fma chain per row in step order -- the reference's lane-wise vtSum.
+// One wave per item.  Wide items: all 64 lanes on one chunk, lane (l%8 = row
+// of the chunk, l/8 = step slice) runs the steps k = kbeg + l/8 (mod 8), so
+// every load instruction reads 64 consecutive entries, and the eight slices
+// are added by a tree at the end (long chunks, where eight lanes alone would
+// serialise thousands of dependent gathers).  Staged blocks: see the file
+// comment; each row's fma chain runs over its steps in order from +0.0 -- the
+// reference's lane-wise vtSum, padding terms (0 * x[0]) included.
 __global__ __launch_bounds__(BLOCK_THREADS) void k1_kernel(
     const K1Item *__restrict__ items, uint32_t nitems, uint32_t nwide,
     const double *__restrict__ val, const uint32_t *__restrict__ col,
     const uint32_t *__restrict__ chp, const uint32_t *__restrict__ permi,
     const double *__restrict__ x, double *__restrict__ y, uint32_t nrows, double *partial)
 {
+    __shared__ double s_v[WAVES][K1_BLK];
+    __shared__ double s_x[WAVES][K1_BLK];
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
     const uint32_t wid = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + w);
+    if (wid >= nitems) return;
     const uint32_t l = lane % 8;
-    constexpr int U = 8;
-    uint32_t it;
-    double acc = 0.0;
     if (wid < nwide) {
-        it = wid;
-        const K1Item d = items[it];
+        constexpr int U = 8;
+        const K1Item d = items[wid];
+        double acc = 0.0;
         const uint32_t base = chp[d.c] + l, s0 = lane / 8;
         for (uint32_t k = d.kbeg + s0; k < d.kend; k += 8 * U) {
             double v[U];
@@ -150,35 +163,59 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k1_kernel(
         acc += shfl_xor_f64(acc, 16);
         acc += shfl_xor_f64(acc, 32);
         if (lane >= 8) return;
-    } else {
-        it = nwide + (wid - nwide) * 8 + lane / 8;
-        if (it >= nitems) return;
-        const K1Item d = items[it];
-        const uint32_t base = chp[d.c] + l;
-        for (uint32_t k = d.kbeg; k < d.kend; k += U) {
-            double v[U];
-            uint32_t cc[U];
+        const uint32_t s = d.c * 8 + l;
+        if (d.k <= 1) {
+            if (!permi) y[s] = acc;
+            else if (s < nrows) y[permi[s]] = acc;
+        } else {
+            partial[(size_t)wid * 8 + l] = acc; // combined by k1_fixup_kernel
+        }
+        return;
+    }
+    const K1Item b = items[wid];
+    const uint32_t c0 = b.c, c1 = b.kbeg, s0 = b.kend, np = (b.b0 - s0) / 2;
+    // phase 1: the block's slots as aligned pairs (s0 and the slot count are
+    // multiples of 8), x gathered, both staged in LDS
+    constexpr int U2 = K1_BLK / (2 * WAVE);
+    if (np != 0) { // uniform
+        d2v v[U2];
+        u2v cc[U2];
 #pragma unroll
-            for (int j = 0; j < U; j++) {
-                uint32_t kk = k + j < d.kend ? k + j : d.kend - 1;
-                v[j] = ntload(val + base + 8 * kk);
-                cc[j] = ntload(col + base + 8 * kk);
+        for (int j = 0; j < U2; j++) {
+            uint32_t q = j * WAVE + lane;
+            q = q < np ? q : np - 1; // branch-free: re-read the last pair
+            v[j] = ntload(reinterpret_cast<const d2v *>(val + s0 + 2 * q));
+            cc[j] = ntload(reinterpret_cast<const u2v *>(col + s0 + 2 * q));
+        }
+        double x0[U2], x1[U2];
+#pragma unroll
+        for (int j = 0; j < U2; j++) {
+            x0[j] = x[cc[j][0]];
+            x1[j] = x[cc[j][1]];
+        }
+#pragma unroll
+        for (int j = 0; j < U2; j++) {
+            const uint32_t q = j * WAVE + lane;
+            if (q < np) {
+                s_v[w][2 * q] = v[j][0];
+                s_v[w][2 * q + 1] = v[j][1];
+                s_x[w][2 * q] = x0[j];
+                s_x[w][2 * q + 1] = x1[j];
             }
-            double xv[U];
-#pragma unroll
-            for (int j = 0; j < U; j++) xv[j] = x[cc[j]];
-#pragma unroll
-            for (int j = 0; j < U; j++)
-                if (k + j < d.kend) acc = fma(v[j], xv[j], acc);
         }
     }
-    const K1Item d = items[it];
-    const uint32_t s = d.c * 8 + l;
-    if (d.k <= 1) {
+    wave_lds_sync();
+    // phase 2: row l of chunk c is slots chp[c] + 8k + l, k < nmc[c]
+    for (uint32_t r = lane; r < 8 * (c1 - c0); r += WAVE) {
+        const uint32_t c = c0 + r / 8, lr = r % 8;
+        const uint32_t a = chp[c], e = c + 1 < c1 ? chp[c + 1] : b.b0;
+        const uint32_t o = a - s0 + lr, n = (e - a) / 8;
+        double acc = 0.0;
+        in_order_chunks<8, d2v>(0, n, [&](uint32_t k) { return d2v{s_v[w][o + 8 * k], s_x[w][o + 8 * k]}; },
+                                [&](d2v t) { acc = fma(t[0], t[1], acc); });
+        const uint32_t s = c * 8 + lr;
         if (!permi) y[s] = acc;
         else if (s < nrows) y[permi[s]] = acc;
-    } else {
-        partial[(size_t)it * 8 + l] = acc; // combined by k1_fixup_kernel
     }
 }
 
@@ -268,8 +305,23 @@ extern "C" int intspmv_k1_plan(intspmv_ctx *ctx, uint32_t nrows, uint32_t chunkN
         if (np > 1) splits.push_back({c, b0, np, 0});
     }
     nwide = (uint32_t)items.size();
-    for (uint32_t c = 0; c < chunkNum; c++)
-        if (nmc_host[c] <= K1_WIDE) items.push_back({c, 0, nmc_host[c], (uint32_t)items.size(), 1, 0, 0, 0});
+    // runs of the other chunks in staged blocks of at most K1_BLK slots and
+    // K1_BLK_CHUNKS chunks (chunk c starts at slot chp[c] = 8 * the steps of
+    // the chunks before it; a narrow chunk, <= 8 * K1_WIDE slots, always fits)
+    {
+        uint32_t c = 0, slot = 0;
+        while (c < chunkNum) {
+            if (nmc_host[c] > K1_WIDE) {
+                slot += 8 * nmc_host[c++];
+                continue;
+            }
+            const uint32_t c0 = c, s0 = slot;
+            while (c < chunkNum && c - c0 < K1_BLK_CHUNKS && nmc_host[c] <= K1_WIDE &&
+                   slot + 8 * nmc_host[c] - s0 <= K1_BLK)
+                slot += 8 * nmc_host[c++];
+            items.push_back({c0, c, s0, slot, 0, 0, 0, 0});
+        }
+    }
     intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));
     if (!p) return INTSPMV_ERR_NOMEM;
     p->ctx = ctx;
@@ -300,11 +352,12 @@ extern "C" int spmv_k1(intspmv_ctx *ctx, const intspmv_plan *plan, const double 
     int st = ispv_enter(ctx);
     if (st) return st;
     if (plan->nblocks == 0) return INTSPMV_OK;
-    // a matrix without entries (every chunk 0 steps) reads no val/col
+    // a matrix without entries (every chunk 0 steps) reads no val/col; the
+    // staged blocks read val and col in 16- and 8-byte pairs
     if (!chp || !x || !y || (plan->nunits && (!val || !col))) return INTSPMV_ERR_ARG;
+    if ((uintptr_t)val % 16 != 0 || (uintptr_t)col % 8 != 0) return INTSPMV_ERR_ARG;
     const uint32_t nwide = plan->nlong_wg;
-    uint32_t waves = nwide + (plan->nblocks - nwide + 7) / 8;
-    dim3 grid((waves + WAVES - 1) / WAVES);
+    dim3 grid((plan->nblocks + WAVES - 1) / WAVES); // one wave per item
     k1_kernel<<<grid, BLOCK_THREADS, 0, ctx->stream>>>(
         (const K1Item *)plan->d_blocks, plan->nblocks, nwide, val, col, chp, permi, x, y,
         plan->nrows, plan->d_partial);
