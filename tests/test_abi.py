@@ -51,3 +51,26 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_loader, "LIBDIR", str(tmp_path))
     with pytest.raises(_loader.NativeLibraryMissing):
         _loader.load_library("libintspmv.so")
+
+
+def test_headers_compile_and_link_as_c(tmp_path):
+    """A C11 host that includes both headers and takes the address of every
+    declared entry point compiles warning-free and links against the two
+    libraries: the boundary a C caller (the reference's own language) binds."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    if not cc:
+        pytest.skip("no C compiler")
+    names = _declared("intspmv.h") + _declared("intspmv_host.h")
+    src = tmp_path / "use.c"
+    src.write_text('#include "intspmv.h"\n#include "intspmv_host.h"\n#include <stdio.h>\n'
+                   "int main(void) {\n    void *p[] = {\n"
+                   + "".join(f"        (void *)&{n},\n" for n in names)
+                   + '    };\n    printf("%zu\\n", sizeof p / sizeof p[0]);\n    return 0;\n}\n')
+    lib = os.path.join(ROOT, "int_spmv_amd", "lib")
+    r = subprocess.run([cc, "-std=c11", "-Wall", "-Wextra", "-Werror", "-Wno-pedantic",
+                        f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(tmp_path / "use"),
+                        f"-L{lib}", "-lintspmv", "-lintspmv_host", "-L/opt/rocm/lib",
+                        "-Wl,--allow-shlib-undefined"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
