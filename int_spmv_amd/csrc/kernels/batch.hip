@@ -58,6 +58,9 @@ struct intspmv_batch {
 
 namespace ispv {
 
+#ifndef ISPV_B_UG1
+#define ISPV_B_UG1 1 // one right-hand side: one group of 16 tiles per step (fewer registers)
+#endif
 template <typename VT, typename XT, typename AT, bool XB>
 __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
     const BatchItem *__restrict__ items, const uint32_t *__restrict__ xcd_range,
@@ -86,10 +89,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
 #define ISPV_B(KC)                                                                              \
     if (md.kc == KC) {                                                                          \
         if (md.th == 1)                                                                         \
-            gather_block<VT, XT, AT, KC, true, XB>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy,  \
+            gather_block<VT, XT, AT, KC, true, XB, ISPV_B_UG1>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy,  \
                                                item.j0, kv, 1, md.nrows, part, L.val, L.key, L.seg); \
         else                                                                                    \
-            gather_block<VT, XT, AT, KC, false, XB>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy, \
+            gather_block<VT, XT, AT, KC, false, XB, ISPV_B_UG1>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy, \
                                                 item.j0, kv, md.th, md.nrows, part, L.val, L.key,  \
                                                 L.seg);                                         \
     }

@@ -20,6 +20,10 @@
 
 namespace ispv {
 
+#ifndef ISPV_G_UG1
+#define ISPV_G_UG1 8 // one right-hand side: 8 groups of 16 tiles, 16 gathers per lane in flight
+#endif
+
 template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
@@ -33,7 +37,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
     GatherLds<AT, KC> &L = lds[threadIdx.x / WAVE];
-    gather_block<VT, XT, AT, KC, H1, XB>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv, th, nrows,
+    gather_block<VT, XT, AT, KC, H1, XB, ISPV_G_UG1>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv, th, nrows,
                                      partial, L.val, L.key, L.seg);
 }
 

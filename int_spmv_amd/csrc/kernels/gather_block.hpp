@@ -132,7 +132,10 @@ struct GatherLds {
     uint16_t seg[G_UMAX + 1];
 };
 
-template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB>
+// UG1: groups of 16 tiles in flight per step for one right-hand side (the
+// single-matrix kernel keeps more gathers in flight; the batched kernel,
+// whose registers are those of its widest path, fewer).
+template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB, int UG1 = 4>
 __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__restrict__ aval,
                                              const uint32_t *__restrict__ acol,
                                              const uint32_t *__restrict__ sec,
@@ -150,7 +153,7 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
     for (int j = 0; j < KC; j++) lacc[j] = 0;
     // tiles of this block; UG steps of 16 tiles are loaded before their
     // gathers are issued, so a lane keeps 2*UG gathers in flight
-    constexpr int UG = KC >= 8 ? 1 : (KC >= 4 ? 2 : 4);
+    constexpr int UG = KC >= 8 ? 1 : (KC >= 4 ? 2 : (KC == 1 ? UG1 : 4));
     const uint32_t T0 = H1 ? d.ubeg : d.ubeg / 8, T1 = H1 ? d.uend : (d.uend + 7) / 8;
     for (uint32_t Tbb = T0; Tbb < T1; Tbb += 16 * UG) {
         AT a0[UG][KC], a1[UG][KC];
