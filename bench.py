@@ -530,7 +530,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": M.HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / M.HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "axt_h1_kernel<double,double,double,false,8> (+ split fix-up)"},
+                     "kernel": "axt_h1_kernel<double,double,double,false> (+ split fix-up)"},
         "cpu_baseline": cpu,
         "formats": formats,
         "batch": batch,
