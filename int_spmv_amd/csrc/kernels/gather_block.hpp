@@ -200,7 +200,6 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
 #pragma unroll
         for (int u = 0; u < UG; u++) {
             const uint32_t Tr = Tbb + 16 * u + lane / 4;
-            const size_t T = Tr < T1 ? Tr : T1 - 1;
             if (H1) {
                 const bool ok = Tr < T1;
                 if (d.k != 0) {
