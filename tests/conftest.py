@@ -32,7 +32,7 @@ def tiny10():
 
 @pytest.fixture(scope="session")
 def lap256():
-    """configs[0]-style input: 256x256 5-point Laplacian, x ~ U[-1,1] seed 0."""
+    """The 256x256 5-point Laplacian (n = 65,536), x ~ U[-1,1] seed 0."""
     from int_spmv_amd import host as H
     m = H.CSR.laplace2d(256)
     return m, H.uniform(m.nrows, -1.0, 1.0, seed=0)

@@ -248,7 +248,7 @@ def test_axt_repeat_is_deterministic(engine, mats):
 
 @pytest.mark.parametrize("bs", [2, 8, 128, 256, 512])
 def test_axt_com(engine, lap256, mats, tiny10, bs):
-    """COM is judged against CSR-naive (the reference's COM kernel drops row 0
+    """COM is checked against CSR-naive (the reference's COM kernel drops row 0
     segments whose head word is 0 and sums by prefix differences); rows that
     fit one block are bit-identical to cf_CSR."""
     fx, tm, tx = tiny10
