@@ -174,9 +174,11 @@ int spmv_axt_grad_vals_embedded(intspmv_ctx *ctx, const intspmv_plan *plan, cons
  * values between the slot orders of A and of A^T (the transposed map). */
 int spmv_permute_vals(intspmv_ctx *ctx, uint32_t n, int val_type, const uint32_t *perm,
                       const void *src, void *dst);
-/* Device-side conversion: rewrites the embedded x of every occupied slot of
- * the reference layout from acol and a new x (the host builder's ax[..+8] =
- * vec[col] step, intSpmv_main.c:1171, 1207), so x can change without a rebuild. */
+/* Device-side conversion: rewrites the embedded x of every slot of the
+ * reference layout from acol and a new x (the host builder's ax[..+8] =
+ * vec[col] step, intSpmv_main.c:1171, 1207; 0 on padding, as the builder
+ * leaves it), so x can change without a rebuild.  acol 8-byte and ax 16-byte
+ * aligned. */
 int spmv_axt_refresh_x(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_t *acol,
                        const double *x, double *ax);
 

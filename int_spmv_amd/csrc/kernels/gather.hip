@@ -112,18 +112,24 @@ __global__ __launch_bounds__(256) void permute_vals_kernel(uint32_t n, const uin
     dst[s] = p != PAD ? src[p] : (VT)0;
 }
 
-// Embedded x refresh: the x half of every occupied slot becomes x[acol[s]].
+// Embedded x refresh: the x half of every slot becomes x[acol[s]], 0 on
+// padding (as the builder leaves it).  Two slots per thread: one 8-byte
+// column load and one 16-byte store (slot counts are multiples of 8).
 __global__ __launch_bounds__(256) void refresh_x_kernel(uint32_t nslots, uint32_t th,
                                                         const uint32_t *__restrict__ acol,
                                                         const double *__restrict__ x,
                                                         double *__restrict__ ax)
 {
-    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t s = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (s >= nslots) return;
-    const uint32_t c = ntload(acol + s);
-    if (c == PAD) return;
+    const u2v c = ntload(reinterpret_cast<const u2v *>(acol + s));
+    const bool ok0 = c[0] != PAD, ok1 = c[1] != PAD;
+    const double x0 = x[ok0 ? c[0] : 0], x1 = x[ok1 ? c[1] : 0]; // row 0 read for padding
+    d2v o;
+    o[0] = ok0 ? x0 : 0.0;
+    o[1] = ok1 ? x1 : 0.0;
     const size_t T = s / (8 * th), w = s % (8 * th);
-    ax[(T * th + w / 8) * 16 + 8 + w % 8] = x[c];
+    *reinterpret_cast<d2v *>(ax + (T * th + w / 8) * 16 + 8 + w % 8) = o;
 }
 
 template <typename VT, typename XT, typename AT>
@@ -292,6 +298,7 @@ extern "C" int spmv_axt_refresh_x(intspmv_ctx *ctx, const intspmv_plan *plan,
     if (!acol || !x || !ax) return INTSPMV_ERR_ARG;
     int st = ispv_enter(ctx);
     if (st) return st;
-    refresh_x_kernel<<<(nslots + 255) / 256, 256, 0, ctx->stream>>>(nslots, D.tileH, acol, x, ax);
+    if (nslots % 8 || (uintptr_t)acol % 8 || (uintptr_t)ax % 16) return INTSPMV_ERR_ARG;
+    refresh_x_kernel<<<(nslots / 2 + 255) / 256, 256, 0, ctx->stream>>>(nslots, D.tileH, acol, x, ax);
     return ispv_launch_check(ctx);
 }

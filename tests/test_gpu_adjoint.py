@@ -179,18 +179,24 @@ def test_adjoint_and_grad_vals(engine, mats, th):
         assert bitwise_equal(_host(vb2, a.lenAX // 2), want), (name, th)
 
 
-def test_refresh_x_on_device(engine, mats):
+@pytest.mark.parametrize("th", [1, 4, 8])
+def test_refresh_x_on_device(engine, mats, th):
+    """The device refresh gives the host rebuild's arrays bit for bit,
+    padding included (x half 0)."""
     for name, m in mats:
         x0, x1 = H.uniform(m.nrows, seed=1), H.uniform(m.nrows, seed=2)
-        a = H.AXT(m, x0, th=4, want_acol=True)
+        a = H.AXT(m, x0, th=th, want_acol=True)
         e = D.DeviceAXT(engine, a)
+        stale = a.ax.copy()
+        stale.reshape(-1, 16)[:, 8:] = np.nan  # every x half must be rewritten
+        e.ax.copy_(engine.upload(stale))
         D.refresh_x(e, engine.upload(a.acol), engine.upload(x1))
         y = _nan(engine, m.nrows)
         e.int_axt(y)
-        b = H.AXT(m, x1, th=4)
+        b = H.AXT(m, x1, th=th)
         import torch
         torch.cuda.synchronize()
-        assert bitwise_equal(e.ax.cpu().numpy(), b.ax), name
+        assert bitwise_equal(e.ax.cpu().numpy(), b.ax), (name, th)
         assert scaled(m, x1, O.cf_csr(m.row, m.col, m.val, x1), _host(y, m.nrows)) <= 1e-12
 
 
