@@ -427,6 +427,11 @@ def main():
         at = adj.at
         record("AXT_ADJOINT_H1_F64", M.bytes_axt_gather(n, at.lenAX, at.lenSEC), w, e,
                {"vs_forward_gather": round(e / fwd_g, 3), "vs_forward_embedded": round(e / head_ev, 3)})
+        # when A's values change (a training step), the transposed map's copy
+        # of them is refreshed once: a permutation of the slot values
+        w, e = timed(lambda: adj.refresh(g.aval), 20, 3, torch)
+        record("AXT_ADJOINT_VALUES_H1_F64", (at.lenAX // 2) * (4 + 8 + 8), w, e,
+               {"vs_forward_gather": round(e / fwd_g, 3)})
         vb = eng.empty(ag.lenAX // 2)
         w, e = timed(lambda: g.grad_vals(xd, yb, vb), 20, 3, torch)
         record("AXT_GRAD_VALS_H1_F64", (ag.lenAX // 2) * (4 + 8) + ag.lenSEC * 4 + 2 * n * 8, w, e,
