@@ -427,8 +427,12 @@ def main():
         at = adj.at
         record("AXT_ADJOINT_H1_F64", M.bytes_axt_gather(n, at.lenAX, at.lenSEC), w, e,
                {"vs_forward_gather": round(e / fwd_g, 3), "vs_forward_embedded": round(e / head_ev, 3)})
-        # when A's values change (a training step), the transposed map's copy
-        # of them is refreshed once: a permutation of the slot values
+        # right after a value update (a training step): A's values read through
+        # the slot map, or first copied into Aᵀ's order (a permutation)
+        w, e = timed(lambda: adj.adjoint(yb, xb, vals=g.aval), 20, 3, torch)
+        record("AXT_ADJOINT_PERMUTED_H1_F64",
+               M.bytes_axt_gather(n, at.lenAX, at.lenSEC) + (at.lenAX // 2) * 4, w, e,
+               {"vs_forward_gather": round(e / fwd_g, 3)})
         w, e = timed(lambda: adj.refresh(g.aval), 20, 3, torch)
         record("AXT_ADJOINT_VALUES_H1_F64", (at.lenAX // 2) * (4 + 8 + 8), w, e,
                {"vs_forward_gather": round(e / fwd_g, 3)})

@@ -162,6 +162,16 @@ int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval
 /* kc: right-hand sides per pass over the matrix (1, 2, 4 or 8, and 16 for F32
  * and BF16 values); 0 = for each pass the largest of these <= what is left
  * of k (16 only for F32/BF16). */
+/* The same product with the slot values read through a permutation: value
+ * of slot s = vals[vperm[s]] (0 where vperm[s] = 0xFFFFFFFF; vperm 8-byte
+ * aligned).  On the transposed tile map, with its slot map (A^T slot -> A
+ * slot) and A's own values, this is the adjoint X_bar = A^T Y_bar straight
+ * after a value update, without first moving the values into A^T's slot
+ * order (spmv_permute_vals).  F64 and F32.  No reference counterpart. */
+int spmv_axt_gather_permuted(intspmv_ctx *ctx, const intspmv_plan *plan, const void *vals,
+                             const uint32_t *vperm, const uint32_t *acol, const uint32_t *sec,
+                             const void *x, uint32_t ldx, void *y, uint32_t ldy, uint32_t k,
+                             uint32_t kc);
 /* Value gradient on the pattern: vbar[s] = ybar[row(s)] * x[acol[s]] (0 on
  * padding), one entry per slot (lenAX/2).  F64 or F32. */
 int spmv_axt_grad_vals(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_t *acol,

@@ -1,7 +1,8 @@
 """torch.autograd.Function over the C ABI (used by the test harness).
 
 y = A(vals) x with A an AXT NOC matrix in column-map form.  Backward:
-  x_bar    = A^T y_bar   -- spmv_axt_gather on the transposed tile map
+  x_bar    = A^T y_bar   -- spmv_axt_gather_permuted on the transposed tile map,
+                           reading A's current values through the slot map
   vals_bar = y_bar[row(s)] * x[col(s)] per slot -- spmv_axt_grad_vals
 Both are hand-written gfx950 kernels; torch only holds the tensors.
 """
@@ -46,9 +47,8 @@ class AxtSpMV(torch.autograd.Function):
         ybar = ybar.contiguous()
         gx = gv = None
         if ctx.needs_input_grad[1]:
-            op.adj.refresh(vals)
             gx = torch.empty_like(x)
-            op.adj.adjoint(ybar, gx)
+            op.adj.adjoint(ybar, gx, vals=vals)  # A's values read through the slot map
         if ctx.needs_input_grad[0]:
             gv = torch.empty_like(vals)
             op.fwd.grad_vals(x, ybar, gv)

@@ -24,21 +24,22 @@ namespace ispv {
 #define ISPV_G_UG1 8 // one right-hand side: 8 groups of 16 tiles, 16 gathers per lane in flight
 #endif
 
-template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB>
+template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB, bool PV>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
     const VT *__restrict__ aval, const uint32_t *__restrict__ acol,
     const uint32_t *__restrict__ sec, const XT *__restrict__ x, uint32_t ldx,
     AT *__restrict__ y, uint32_t ldy, uint32_t j0, uint32_t kv, uint32_t th, uint32_t nrows,
-    double *partial)
+    double *partial, const uint32_t *__restrict__ vperm)
 {
     __shared__ GatherLds<AT, KC> lds[WAVES];
     const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
     GatherLds<AT, KC> &L = lds[threadIdx.x / WAVE];
-    gather_block<VT, XT, AT, KC, H1, XB, ISPV_G_UG1>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv, th, nrows,
-                                     partial, L.val, L.key, L.seg);
+    gather_block<VT, XT, AT, KC, H1, XB, ISPV_G_UG1, PV>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv,
+                                                       th, nrows, partial, L.val, L.key, L.seg,
+                                                       vperm);
 }
 
 // A_bar[s] = y_bar[row(s)] * x[acol[s]] (0 on padding).  row(s) is the sec
@@ -132,10 +133,10 @@ __global__ __launch_bounds__(256) void refresh_x_kernel(uint32_t nslots, uint32_
     *reinterpret_cast<d2v *>(ax + (T * th + w / 8) * 16 + 8 + w % 8) = o;
 }
 
-template <typename VT, typename XT, typename AT>
+template <typename VT, typename XT, typename AT, bool PV = false>
 static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t *acol,
                          const uint32_t *sec, const void *x, uint32_t ldx, void *y, uint32_t ldy,
-                         uint32_t k, uint32_t kcf)
+                         uint32_t k, uint32_t kcf, const uint32_t *vperm = nullptr)
 {
     const uint32_t nb = p->nblocks, th = p->axt.tileH;
     if (nb == 0) return INTSPMV_OK;
@@ -153,9 +154,9 @@ static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t
         const uint32_t kv = rem < kc ? rem : kc;
         const bool xb = xb_ok && kc < 16 / sizeof(XT);
 #define ISPV_GX(KC, H1, XB)                                                                     \
-    axt_gather_kernel<VT, XT, AT, KC, H1, XB><<<grid, blk, 0, s>>>(                             \
+    axt_gather_kernel<VT, XT, AT, KC, H1, XB, PV><<<grid, blk, 0, s>>>(                         \
         p->d_blocks, nl, nb, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y, ldy, j0, \
-        kv, th, p->nrows, p->d_partial)
+        kv, th, p->nrows, p->d_partial, vperm)
 #define ISPV_G(KC)                                                                              \
     if (kc == KC) {                                                                             \
         if constexpr (KC < 16 / sizeof(XT)) {                                                   \
@@ -219,6 +220,32 @@ extern "C" int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const
         return launch_gather<float, float, float>(plan, aval, acol, sec, x, ldx, y, ldy, k, kc);
     if (D.val_type == INTSPMV_BF16 && D.x_type == INTSPMV_F32)
         return launch_gather<uint16_t, float, float>(plan, aval, acol, sec, x, ldx, y, ldy, k, kc);
+    return INTSPMV_ERR_UNSUPPORTED;
+}
+
+extern "C" int spmv_axt_gather_permuted(intspmv_ctx *ctx, const intspmv_plan *plan,
+                                        const void *vals, const uint32_t *vperm,
+                                        const uint32_t *acol, const uint32_t *sec, const void *x,
+                                        uint32_t ldx, void *y, uint32_t ldy, uint32_t k,
+                                        uint32_t kc)
+{
+    if (!gather_plan_ok(ctx, plan) || plan->umax > G_UMAX || k == 0 || k > PARTIAL_STRIDE ||
+        ldx < k || ldy < k ||
+        (kc != 0 && kc != 1 && kc != 2 && kc != 4 && kc != 8 &&
+         !(kc == 16 && plan->axt.val_type != INTSPMV_F64)))
+        return INTSPMV_ERR_ARG;
+    if (plan->nrows && (!x || !y || (plan->nunits && (!vals || !vperm || !acol || !sec))))
+        return INTSPMV_ERR_ARG;
+    if (plan->nunits && (uintptr_t)vperm % 8 != 0) return INTSPMV_ERR_ARG; // pairs of indexes
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    const intspmv_axt_desc &D = plan->axt;
+    if (D.val_type == INTSPMV_F64 && D.x_type == INTSPMV_F64)
+        return launch_gather<double, double, double, true>(plan, vals, acol, sec, x, ldx, y, ldy,
+                                                           k, kc, vperm);
+    if (D.val_type == INTSPMV_F32 && D.x_type == INTSPMV_F32)
+        return launch_gather<float, float, float, true>(plan, vals, acol, sec, x, ldx, y, ldy, k,
+                                                        kc, vperm);
     return INTSPMV_ERR_UNSUPPORTED;
 }
 

@@ -135,7 +135,10 @@ struct GatherLds {
 // UG1: groups of 16 tiles in flight per step for one right-hand side (the
 // single-matrix kernel keeps more gathers in flight; the batched kernel,
 // whose registers are those of its widest path, fewer).
-template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB, int UG1 = 4>
+// PV: slot values read through a permutation, value of slot s =
+// aval[vperm[s]] (0 where vperm[s] is PAD) -- Aᵀ's map reading A's values.
+template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB, int UG1 = 4,
+          bool PV = false>
 __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__restrict__ aval,
                                              const uint32_t *__restrict__ acol,
                                              const uint32_t *__restrict__ sec,
@@ -143,7 +146,7 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                                              AT *__restrict__ y, uint32_t ldy, uint32_t j0,
                                              uint32_t kv, uint32_t th, uint32_t nrows,
                                              double *partial, AT *sv, uint32_t *sk,
-                                             uint16_t *sg)
+                                             uint16_t *sg, const uint32_t *__restrict__ vperm = nullptr)
 {
     const int lane = lane_id();
     const uint32_t nu = d.uend - d.ubeg, q = lane & 3, C = 2 * q;
@@ -177,7 +180,14 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                 const uint32_t Tr = Tbb + 16 * u + lane / 4;
                 const size_t T = Tr < T1 ? Tr : T1 - 1;
                 const size_t sidx = (T * th + R) * 8 + C;
-                GPair<VT>::ld(aval + sidx, v0[u], v1[u]);
+                if constexpr (PV) {
+                    static_assert(!std::is_same<VT, uint16_t>::value, "FP64 / FP32 values");
+                    const u2v pp = ntload(reinterpret_cast<const u2v *>(vperm + sidx));
+                    v0[u] = pp[0] != PAD ? aval[pp[0]] : (VT)0;
+                    v1[u] = pp[1] != PAD ? aval[pp[1]] : (VT)0;
+                } else {
+                    GPair<VT>::ld(aval + sidx, v0[u], v1[u]);
+                }
                 cc[u] = ntload(reinterpret_cast<const u2v *>(acol + sidx));
             }
 #pragma unroll

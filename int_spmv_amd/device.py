@@ -74,6 +74,8 @@ SIGNATURES = [
     ("intspmv_axt_plan", C.c_int, [_vp, C.POINTER(AxtDesc), _vp, C.POINTER(_vp)]),
     ("spmv_axt", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_axt_gather", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32, _u32]),
+    ("spmv_axt_gather_permuted", C.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32, _u32]),
     ("spmv_axt_grad_vals", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_axt_grad_vals_embedded", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_permute_vals", C.c_int, [_vp, _u32, C.c_int, _vp, _vp, _vp]),
@@ -448,8 +450,17 @@ class AxtAdjoint:
                                                 _dptr(self.permT), _dptr(vals),
                                                 _dptr(self.gT.aval)), "spmv_permute_vals")
 
-    def adjoint(self, ybar, xbar, k: int = 1) -> None:
-        self.gT.matvec(ybar, xbar, k)
+    def adjoint(self, ybar, xbar, k: int = 1, vals=None) -> None:
+        """X_bar = A^T Y_bar.  With vals (A's slot values, device) the kernel
+        reads them through the slot map, so no refresh is needed after a
+        value update; otherwise it uses the copy made by refresh()."""
+        if vals is None:
+            self.gT.matvec(ybar, xbar, k)
+            return
+        g = self.gT
+        self.eng._check(lib().spmv_axt_gather_permuted(
+            self.eng.h, g.plan.h, _dptr(vals), _dptr(self.permT), _dptr(g.acol), _dptr(g.sec),
+            _dptr(ybar), k, _dptr(xbar), k, k, 0), "spmv_axt_gather_permuted")
 
 
 def H_AXT(m, x, **kw):

@@ -165,6 +165,15 @@ def test_adjoint_and_grad_vals(engine, mats, th):
         ref = O.csr_transpose_product(m.row, m.col, m.val, ybar)
         at, _ = m.transpose()
         assert scaled(at, ybar, ref, got) <= 1e-12, (name, th)
+        # A's values read through the slot map: the same bits, k = 1 and 3
+        xb2 = _nan(engine, n)
+        adj.adjoint(engine.upload(ybar), xb2, vals=fwd.aval)
+        assert bitwise_equal(_host(xb2, n), got), (name, th)
+        Yb = np.stack([H.normal(n, seed=20 + j) for j in range(3)], axis=1)
+        Xr, Xp = _nan(engine, 3 * n), _nan(engine, 3 * n)
+        adj.adjoint(engine.upload(Yb.reshape(-1)), Xr, k=3)
+        adj.adjoint(engine.upload(Yb.reshape(-1)), Xp, k=3, vals=fwd.aval)
+        assert bitwise_equal(_host(Xp, 3 * n), _host(Xr, 3 * n)), (name, th)
         vb = _nan(engine, a.lenAX // 2)
         fwd.grad_vals(engine.upload(x), engine.upload(ybar), vb)
         g = _host(vb, a.lenAX // 2)
