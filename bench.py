@@ -35,6 +35,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--workload", default="powerlaw_1m")
     p.add_argument("--quick", action="store_true", help="headline kernel only (profiling)")
+    p.add_argument("--batch-only", action="store_true", help="the batched launch only (profiling)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--batch-count", type=int, default=256,
                    help="matrices in the batched-launch measurement (0 = skip)")
@@ -315,6 +316,11 @@ def main():
     from int_spmv_amd import host as H
     from int_spmv_amd import metrics as M
     from int_spmv_amd.workloads import WORKLOADS
+    if args.batch_only:  # the batched launch alone (profiling)
+        eng = D.Engine(0)
+        r = bench_batch(eng, args.batch_count, torch, log, args.batch_tune)
+        print(json.dumps({"batch": r}), flush=True)
+        return
 
     t = time.perf_counter()
     m, x = WORKLOADS[args.workload]()
