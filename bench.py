@@ -442,6 +442,9 @@ def main():
         w, e = timed(lambda: adj.refresh(g.aval), 20, 3, torch)
         record("AXT_ADJOINT_VALUES_H1_F64", (at.lenAX // 2) * (4 + 8 + 8), w, e,
                {"vs_forward_gather": round(e / fwd_g, 3)})
+        # which of the two the autograd backward takes on this matrix
+        formats["AXT_ADJOINT_VALUES_PATH_H1_F64"] = adj.tune_values_path()
+        log(f"[bench] AXT_ADJOINT_VALUES_PATH_H1_F64: {formats['AXT_ADJOINT_VALUES_PATH_H1_F64']}")
         vb = eng.empty(ag.lenAX // 2)
         w, e = timed(lambda: g.grad_vals(xd, yb, vb), 20, 3, torch)
         record("AXT_GRAD_VALS_H1_F64", (ag.lenAX // 2) * (4 + 8) + ag.lenSEC * 4 + 2 * n * 8, w, e,

@@ -3,6 +3,8 @@
 y = A(vals) x with A an AXT NOC matrix in column-map form.  Backward:
   x_bar    = A^T y_bar   -- spmv_axt_gather_permuted on the transposed tile map,
                            reading A's current values through the slot map
+                           (or spmv_permute_vals + the plain product, where
+                           tune_adjoint finds that faster)
   vals_bar = y_bar[row(s)] * x[col(s)] per slot -- spmv_axt_grad_vals
 Both are hand-written gfx950 kernels; torch only holds the tensors.
 """
@@ -24,6 +26,11 @@ class AxtOperator:
         self.fwd = D.DeviceAXTGather(eng, self.a, val_type, val_type)
         self.adj = D.AxtAdjoint(eng, m, self.fwd)
         self.dtype = torch.float64 if val_type == D.F64 else torch.float32
+
+    def tune_adjoint(self) -> dict:
+        """Picks, for this matrix, how backward's x_bar takes the current
+        values (through the slot map, or a refresh of A^T's copy first)."""
+        return self.adj.tune_values_path()
 
     def initial_values(self):
         """The matrix values in slot order (a leaf tensor to optimise)."""
@@ -48,7 +55,7 @@ class AxtSpMV(torch.autograd.Function):
         gx = gv = None
         if ctx.needs_input_grad[1]:
             gx = torch.empty_like(x)
-            op.adj.adjoint(ybar, gx, vals=vals)  # A's values read through the slot map
+            op.adj.adjoint(ybar, gx, vals=vals)  # follows A's current values
         if ctx.needs_input_grad[0]:
             gv = torch.empty_like(vals)
             op.fwd.grad_vals(x, ybar, gv)

@@ -450,10 +450,20 @@ class AxtAdjoint:
                                                 _dptr(self.permT), _dptr(vals),
                                                 _dptr(self.gT.aval)), "spmv_permute_vals")
 
+    # how adjoint(vals=...) takes A's new values: "permuted" reads them
+    # through the slot map inside the product; "refresh" first copies them into
+    # A^T's slot order (refresh) and then runs the plain product.  The two give
+    # the same bits; which is faster depends on the matrix (tune_values_path).
+    values_path = "permuted"
+
     def adjoint(self, ybar, xbar, k: int = 1, vals=None) -> None:
-        """X_bar = A^T Y_bar.  With vals (A's slot values, device) the kernel
-        reads them through the slot map, so no refresh is needed after a
-        value update; otherwise it uses the copy made by refresh()."""
+        """X_bar = A^T Y_bar.  With vals (A's slot values, device) the result
+        follows those values, by the path values_path names (after the
+        "refresh" path A^T's copy holds them too); otherwise it uses the copy
+        made by the last refresh()."""
+        if vals is not None and self.values_path == "refresh":
+            self.refresh(vals)
+            vals = None
         if vals is None:
             self.gT.matvec(ybar, xbar, k)
             return
@@ -461,6 +471,35 @@ class AxtAdjoint:
         self.eng._check(lib().spmv_axt_gather_permuted(
             self.eng.h, g.plan.h, _dptr(vals), _dptr(self.permT), _dptr(g.acol), _dptr(g.sec),
             _dptr(ybar), k, _dptr(xbar), k, k, 0), "spmv_axt_gather_permuted")
+
+
+    def tune_values_path(self, k: int = 1, reps: int = 10) -> dict:
+        """Times adjoint(vals=...) by both paths on this matrix (A's current
+        values, random Y_bar with k columns, on the engine's stream) and keeps
+        the faster one in values_path.  Returns both times in microseconds."""
+        torch = _torch()
+        n_out = self.at.nrows
+        n_in = self.fwd.a.nrows
+        dt = torch.float64 if self.fwd.val_type == F64 else torch.float32
+        yb = torch.rand(n_in * k, dtype=dt, device=self.eng.torch_device)
+        xb = torch.empty(max(n_out * k, 1), dtype=dt, device=self.eng.torch_device)
+        vals = self.fwd.aval
+        times = {}
+        for path in ("permuted", "refresh"):
+            self.values_path = path
+            for _ in range(2):
+                self.adjoint(yb, xb, k, vals=vals)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(self.eng.stream)
+            for _ in range(reps):
+                self.adjoint(yb, xb, k, vals=vals)
+            e1.record(self.eng.stream)
+            e1.synchronize()
+            times[path] = e0.elapsed_time(e1) * 1e3 / reps
+        self.values_path = min(times, key=times.get)
+        return dict(path=self.values_path, permuted_us=round(times["permuted"], 2),
+                    refresh_us=round(times["refresh"], 2))
 
 
 def H_AXT(m, x, **kw):

@@ -188,6 +188,37 @@ def test_adjoint_and_grad_vals(engine, mats, th):
         assert bitwise_equal(_host(vb2, a.lenAX // 2), want), (name, th)
 
 
+@pytest.mark.parametrize("vt", [D.F64, D.F32])
+def test_adjoint_values_paths(engine, mats, vt):
+    """After a value update adjoint(vals=...) follows the new values by either
+    path, with the same bits: A's values doubled (an exact scaling in binary
+    floating point) give exactly twice the adjoint of the stored values.
+    tune_values_path keeps one of the two paths; the result does not change."""
+    for name, m in mats:
+        n = m.nrows
+        dt = np.float64 if vt == D.F64 else np.float32
+        a = H.AXT(m, np.zeros(n), th=4, want_acol=True)
+        fwd = D.DeviceAXTGather(engine, a, vt, vt)
+        adj = D.AxtAdjoint(engine, m, fwd)
+        yb = engine.upload(H.normal(n, seed=13).astype(dt))
+        base = _nan(engine, n, dt)
+        adj.adjoint(yb, base)
+        want = 2 * _host(base, n)
+        v2 = fwd.aval * 2
+        for path in ("permuted", "refresh"):
+            adj.values_path = path
+            adj.refresh(fwd.aval)  # A^T's copy back at the stored values
+            xb = _nan(engine, n, dt)
+            adj.adjoint(yb, xb, vals=v2)
+            assert bitwise_equal(_host(xb, n), want), (name, vt, path)
+        adj.refresh(fwd.aval)
+        t = adj.tune_values_path()
+        assert t["path"] in ("permuted", "refresh") and adj.values_path == t["path"], t
+        xb = _nan(engine, n, dt)
+        adj.adjoint(yb, xb, vals=v2)
+        assert bitwise_equal(_host(xb, n), want), (name, vt, t)
+
+
 @pytest.mark.parametrize("th", [1, 4, 8])
 def test_refresh_x_on_device(engine, mats, th):
     """The device refresh gives the host rebuild's arrays bit for bit,
