@@ -308,8 +308,10 @@ def main():
         src = torch.ones(n, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         w, e = timed(lambda: dst.copy_(src), args.steps, args.warmup, torch)
+        # a read-only stream (torch's sum) for the read-dominated kernels' ceiling
+        w2, e2 = timed(lambda: torch.sum(src), args.steps, args.warmup, torch)
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
-                          "GBs_rw": 2 * n * 8 / e / 1e9}))
+                          "GBs_rw": 2 * n * 8 / e / 1e9, "sum_GBs_read": n * 8 / e2 / 1e9}))
         return
 
     from int_spmv_amd import device as D
