@@ -2,6 +2,8 @@
 independent numpy restatement; I/O round trips; error paths."""
 import numpy as np
 import pytest
+from hypothesis import given, settings
+from hypothesis import strategies as st
 
 from int_spmv_amd import host as H
 from oracle import builders_np as B
@@ -71,6 +73,42 @@ def test_c_builders_equal_numpy(name, m):
         assert np.array_equal(a.ax, b["ax"]), (name, mode, th, bs)
         assert np.array_equal(a.sec, b["sec"]), (name, mode, th, bs)
         assert np.array_equal(a.con, b["con"]), (name, mode, th, bs)
+        assert (a.tileN, a.lenAX, a.lenSEC, a.colN) == (b["tileN"], b["lenAX"], b["lenSEC"], b["colN"])
+    ax, brp = B.axc(row, col, val, x)
+    c = H.AXC(m, x)
+    assert np.array_equal(c.ax, ax) and np.array_equal(c.brp, brp)
+    k = H.K1(m)
+    kb = B.k1(row, col, val)
+    for f in ("permi", "nmc", "chp", "val", "col"):
+        assert np.array_equal(getattr(k, f), kb[f]), f
+    assert (k.chunkNum, k.lenVC) == (kb["chunkNum"], kb["lenVC"])
+
+
+@settings(max_examples=400, deadline=None, derandomize=True)
+@given(seed=st.integers(0, 2**31 - 1), n=st.integers(1, 160),
+       th=st.sampled_from([1, 2, 3, 4, 5, 7, 8, 16, 32, 48]),
+       bs=st.sampled_from([2, 4, 8, 32, 128, 512]), long_row=st.booleans())
+def test_c_builders_equal_numpy_random(seed, n, th, bs, long_row):
+    """Random ragged matrices (runs of empty rows, duplicate columns, one
+    long row): the C builders and the numpy restatement give the same arrays
+    for NOC at this tile height, COM at this block size, AXC and K1."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 14, size=n)
+    lens[rng.random(n) < 0.25] = 0
+    if long_row:
+        lens[rng.integers(0, n)] = rng.integers(40, 600)
+    row = np.zeros(n + 1, np.uint32)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.integers(0, n, size=k)) for k in lens] +
+                         [np.zeros(0, np.int64)]).astype(np.uint32)
+    val = rng.standard_normal(len(col))
+    m = H.CSR.from_arrays(row, col, val)
+    x = rng.uniform(-1, 1, n)
+    for mode, t in (("NOC", th), ("COM", 1)):
+        a = H.AXT(m, x, th=t, mode=mode, bs=bs, want_acol=True)
+        b = B.axt(row, col, val, x, th=t, mode=mode, bs=bs)
+        for f in ("ax", "sec", "con"):
+            assert np.array_equal(getattr(a, f), b[f]), (mode, t, bs, f)
         assert (a.tileN, a.lenAX, a.lenSEC, a.colN) == (b["tileN"], b["lenAX"], b["lenSEC"], b["colN"])
     ax, brp = B.axc(row, col, val, x)
     c = H.AXC(m, x)

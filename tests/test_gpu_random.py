@@ -1,0 +1,106 @@
+"""GPU: seeded random ragged matrices through every product entry point.
+
+Each seed draws a matrix of 1..3000 rows with runs of empty rows, duplicate
+columns and, for most seeds, one long row (up to 6000 entries, so plans split
+it across waves), a tile height and a COM block size.  Bar, against the
+oracle's CSR-naive (cf_CSR): the ordered CSR product bit-exact; K1's chunk
+results bit-exact for chunks of at most 64 steps; every other product within
+1e-12 * sum_j |a_ij x_j| per row (FP64).  y is pre-filled with NaN so a row
+a kernel forgets to write fails.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from int_spmv_amd import device as D
+from int_spmv_amd import host as H
+
+from .util import bitwise_equal, scaled
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = range(96)
+
+
+def _case(seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.choice([1, 2, 7, 64, 257, 1000, 3000]))
+    lens = rng.integers(0, 30, size=n)
+    lens[rng.random(n) < 0.2] = 0
+    if n > 8:
+        a = int(rng.integers(0, n - 8))
+        lens[a:a + int(rng.integers(1, 8))] = 0  # a run of empty rows
+    if seed % 4 != 3:
+        lens[int(rng.integers(0, n))] = int(rng.integers(100, 6000))
+    row = np.zeros(n + 1, np.uint32)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.integers(0, n, size=k)) for k in lens] +
+                         [np.zeros(0, np.int64)]).astype(np.uint32)
+    m = H.CSR.from_arrays(row, col, rng.standard_normal(len(col)))
+    x = rng.uniform(-1, 1, n)
+    th = int(rng.choice([1, 2, 3, 4, 8, 16, 32, 48]))
+    bs = int(rng.choice([2, 8, 128, 256, 512]))
+    return m, x, th, bs
+
+
+def _y(eng, n, dtype=np.float64):
+    t = eng.empty(n, dtype)
+    t.fill_(float("nan"))
+    return t
+
+
+def _host(t, n):
+    import torch
+    torch.cuda.synchronize()
+    return t.reshape(-1)[:n].double().cpu().numpy()
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_random_matrix_all_products(engine, seed):
+    m, x, th, bs = _case(seed)
+    n = m.nrows
+    print(f"[random] seed={seed} n={n} nnz={m.nnz} th={th} bs={bs}", flush=True)
+    yr = O.cf_csr(m.row, m.col, m.val, x)
+    xd = engine.upload(x)
+    # CSR: ordered (the oracle's order) and blocked
+    d = D.DeviceCSR(engine, m)
+    y1, y2 = _y(engine, n), _y(engine, n)
+    d.cf_csr(xd, y1)
+    d.spmv(xd, y2)
+    assert bitwise_equal(_host(y1, n), yr), seed
+    assert scaled(m, x, yr, _host(y2, n)) <= 1e-12, seed
+    # AXC
+    a = H.AXC(m, x)
+    y = _y(engine, n)
+    D.DeviceAXC(engine, a).int_axc(y)
+    assert scaled(m, x, yr, _host(y, n)) <= 1e-12, seed
+    # K1: chunk results, reordered y, and the fused product
+    k = H.K1(m)
+    dk = D.DeviceK1(engine, k)
+    res, yk, yp = _y(engine, k.chunkNum * 8), _y(engine, n), _y(engine, n)
+    dk.int_k1(xd, res)
+    dk.order(res, yk)
+    dk.int_k1p(xd, yp)
+    ref_res = O.int_k1(k.chunkNum, k.val, k.col, k.chp, k.nmc, x)
+    narrow = ~np.repeat(k.nmc > 64, 8)
+    assert bitwise_equal(_host(res, k.chunkNum * 8)[narrow], ref_res[narrow]), seed
+    assert scaled(m, x, yr, _host(yk, n)) <= 1e-12, seed
+    assert scaled(m, x, yr, _host(yp, n)) <= 1e-12, seed
+    # AXT NOC at this tile height (x embedded), and COM at this block size
+    for mode, t in (("NOC", th), ("COM", 1)):
+        ax = H.AXT(m, x, th=t, mode=mode, bs=bs)
+        y = _y(engine, n)
+        D.DeviceAXT(engine, ax).int_axt(y)
+        assert scaled(m, x, yr, _host(y, n)) <= 1e-12, (seed, mode)
+    # column map, 1 and 3 right-hand sides
+    ag = H.AXT(m, np.zeros(n), th=th, want_acol=True)
+    g = D.DeviceAXTGather(engine, ag)
+    y = _y(engine, n)
+    g.matvec(xd, y)
+    assert scaled(m, x, yr, _host(y, n)) <= 1e-12, seed
+    X = np.stack([x, -x, np.linspace(-1, 1, n)], axis=1)
+    Y = _y(engine, 3 * n)
+    g.matvec(engine.upload(X.reshape(-1)), Y, k=3)
+    Yg = _host(Y, 3 * n).reshape(n, 3)
+    for j in range(3):
+        assert scaled(m, X[:, j], O.cf_csr(m.row, m.col, m.val, X[:, j]), Yg[:, j]) <= 1e-12, (seed, j)
