@@ -1,4 +1,5 @@
-"""GPU: seeded random ragged matrices through every product entry point.
+"""GPU: seeded random ragged matrices through every product entry point,
+and random mixes of them in one batched launch.
 
 Each seed draws a matrix of 1..3000 rows with runs of empty rows, duplicate
 columns and, for most seeds, one long row (up to 6000 entries, so plans split
@@ -104,3 +105,44 @@ def test_random_matrix_all_products(engine, seed):
     Yg = _host(Y, 3 * n).reshape(n, 3)
     for j in range(3):
         assert scaled(m, X[:, j], O.cf_csr(m.row, m.col, m.val, X[:, j]), Yg[:, j]) <= 1e-12, (seed, j)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_batch(engine, seed):
+    """One batched launch over 1..12 of the random matrices, each with its own
+    tile height, 1..32 right-hand sides, a leading dimension of x up to two
+    past k, a drawn RHS blocking kc and units per wave block; FP64 (even
+    seeds, 1e-12) or FP32 storage and sums (odd seeds, 1e-6 against CSR-naive
+    of the FP32-rounded values)."""
+    import torch
+    rng = np.random.default_rng(5000 + seed)
+    f64 = seed % 2 == 0
+    vt, dt, tol = (D.F64, torch.float64, 1e-12) if f64 else (D.F32, torch.float32, 1e-6)
+    dev = engine.torch_device
+    ents = []
+    for i in range(int(rng.integers(1, 13))):
+        m, _, th, _ = _case(int(rng.integers(0, 1 << 20)))
+        n = m.nrows
+        k = int(rng.choice([1, 2, 3, 5, 8, 13, 32]))
+        ldx = k + int(rng.integers(0, 3))
+        X = rng.uniform(-1, 1, (n, ldx))
+        if not f64:
+            X = X.astype(np.float32).astype(np.float64)
+        a = H.AXT(m, np.zeros(n), th=th, want_acol=True)
+        ents.append(dict(a=a, X=torch.tensor(X.reshape(-1), dtype=dt, device=dev),
+                         Y=torch.full((n * k,), float("nan"), dtype=dt, device=dev), k=k, ldx=ldx,
+                         kc=int(rng.choice([0, 1, 2, 4, 8])), umax=int(rng.choice([0, 16, 32, 64, 128])),
+                         m=m, Xh=X[:, :k]))
+    print(f"[random batch] seed={seed} " + " ".join(
+        f"(n={e['m'].nrows} th={e['a'].tileH} k={e['k']} ldx={e['ldx']} kc={e['kc']} umax={e['umax']})"
+        for e in ents), flush=True)
+    b = D.DeviceBatch(engine, ents, vt, vt)
+    b.launch()
+    torch.cuda.synchronize()
+    for i, e in enumerate(ents):
+        m, k = e["m"], e["k"]
+        Y = e["Y"].double().cpu().numpy().reshape(m.nrows, k)
+        vals = m.val if f64 else m.val.astype(np.float32).astype(np.float64)
+        for j in range(k):
+            yr = O.cf_csr(m.row, m.col, vals, e["Xh"][:, j])
+            assert scaled(m, e["Xh"][:, j], yr, Y[:, j]) <= tol, (seed, i, j)
