@@ -1,5 +1,6 @@
 """GPU: seeded random ragged matrices through every product entry point,
-and random mixes of them in one batched launch.
+and random mixes of them in one batched launch; the precision kernels, the
+adjoint and the value gradient on them.
 
 Each seed draws a matrix of 1..3000 rows with runs of empty rows, duplicate
 columns and, for most seeds, one long row (up to 6000 entries, so plans split
@@ -146,3 +147,50 @@ def test_random_batch(engine, seed):
         for j in range(k):
             yr = O.cf_csr(m.row, m.col, vals, e["Xh"][:, j])
             assert scaled(m, e["Xh"][:, j], yr, Y[:, j]) <= tol, (seed, i, j)
+
+
+@pytest.mark.parametrize("seed", range(32))
+def test_random_precision_and_adjoint(engine, seed):
+    """The random matrices through the FP32/BF16 embedded kernels (1e-6 /
+    1e-2), the FP32 column map with 20 right-hand sides (a pass of 16 and a
+    pass of 4, 1e-6), the adjoint on the transposed tile map (1e-12 against
+    the sequential CSR transpose product; the slot-map read gives the same
+    bits) and the value gradient (bit-exact: one product per slot)."""
+    import torch
+    m, x, th, _ = _case(seed)
+    n = m.nrows
+    print(f"[random precision/adjoint] seed={seed} n={n} nnz={m.nnz} th={th}", flush=True)
+    dev = engine.torch_device
+    yr = O.cf_csr(m.row, m.col, m.val, x)
+    a = H.AXT(m, x, th=th)
+    for vt, tol in ((D.F32, 1e-6), (D.BF16, 1e-2)):
+        y = _y(engine, n, np.float32)
+        D.DeviceAXT(engine, a, val_type=vt, x_type=D.F32).int_axt(y)
+        assert scaled(m, x, yr, _host(y, n)) <= tol, (seed, vt)
+    ag = H.AXT(m, np.zeros(n), th=th, want_acol=True)
+    g32 = D.DeviceAXTGather(engine, ag, D.F32, D.F32)
+    k = 20
+    X = np.random.default_rng(seed).uniform(-1, 1, (n, k)).astype(np.float32).astype(np.float64)
+    Y = torch.full((n * k,), float("nan"), dtype=torch.float32, device=dev)
+    g32.matvec(torch.tensor(X.reshape(-1), dtype=torch.float32, device=dev), Y, k=k)
+    Yg = _host(Y, n * k).reshape(n, k)
+    v32 = m.val.astype(np.float32).astype(np.float64)
+    for j in range(k):
+        assert scaled(m, X[:, j], O.cf_csr(m.row, m.col, v32, X[:, j]), Yg[:, j]) <= 1e-6, (seed, j)
+    g = D.DeviceAXTGather(engine, ag)
+    adj = D.AxtAdjoint(engine, m, g)
+    ybar = np.random.default_rng(seed + 1).standard_normal(n)
+    ybd = engine.upload(ybar)
+    xb1, xb2 = _y(engine, n), _y(engine, n)
+    adj.adjoint(ybd, xb1)
+    adj.adjoint(ybd, xb2, vals=g.aval)
+    at, _ = m.transpose()
+    got = _host(xb1, n)
+    assert scaled(at, ybar, O.csr_transpose_product(m.row, m.col, m.val, ybar), got) <= 1e-12, seed
+    assert bitwise_equal(_host(xb2, n), got), seed
+    vb = _y(engine, ag.lenAX // 2)
+    g.grad_vals(engine.upload(x), ybd, vb)
+    rows = np.repeat(np.arange(n), np.diff(m.row.astype(np.int64)))
+    want = np.zeros(ag.lenAX // 2)
+    want[ag.nzslot] = ybar[rows] * x[m.col]
+    assert bitwise_equal(_host(vb, ag.lenAX // 2), want), seed
