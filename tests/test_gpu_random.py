@@ -1,6 +1,7 @@
 """GPU: seeded random ragged matrices through every product entry point,
 and random mixes of them in one batched launch; the precision kernels, the
-adjoint and the value gradient on them.
+adjoint and the value gradient on them; the device-side conversions (K1
+build, x refresh) against the host builders.
 
 Each seed draws a matrix of 1..3000 rows with runs of empty rows, duplicate
 columns and, for most seeds, one long row (up to 6000 entries, so plans split
@@ -194,3 +195,36 @@ def test_random_precision_and_adjoint(engine, seed):
     want = np.zeros(ag.lenAX // 2)
     want[ag.nzslot] = ybar[rows] * x[m.col]
     assert bitwise_equal(_host(vb, ag.lenAX // 2), want), seed
+
+
+@pytest.mark.parametrize("seed", range(32))
+def test_random_device_conversion(engine, seed):
+    """Device-side conversions on the random matrices, bit for bit against the
+    host builders: the K1 build from device CSR (ties of equal row length
+    are many here; the device sort keeps the host's stable order) and the x
+    refresh of an embedded AXT (every x half first set to NaN)."""
+    import torch
+    m, x, th, _ = _case(seed)
+    n = m.nrows
+    print(f"[random conversion] seed={seed} n={n} nnz={m.nnz} th={th}", flush=True)
+    k = H.K1(m)
+    dk = D.DeviceK1.from_device_csr(engine, D.DeviceCSR(engine, m))
+    torch.cuda.synchronize()
+
+    def u32(t, c):
+        return t[:c].cpu().numpy().view(np.uint32)
+
+    assert (dk.chunkNum, dk.lenVC) == (k.chunkNum, k.lenVC), seed
+    assert np.array_equal(u32(dk.permi, (k.chunkNum + 1) * 8), k.permi), seed
+    for f, c in (("nmc", k.chunkNum), ("chp", k.chunkNum), ("col", k.lenVC)):
+        assert np.array_equal(u32(getattr(dk, f), c), getattr(k, f)), (seed, f)
+    assert bitwise_equal(dk.val[: k.lenVC].cpu().numpy(), k.val), seed
+    x1 = np.random.default_rng(seed + 7).uniform(-1, 1, n)
+    a = H.AXT(m, x, th=th, want_acol=True)
+    e = D.DeviceAXT(engine, a)
+    stale = a.ax.copy()
+    stale.reshape(-1, 16)[:, 8:] = np.nan
+    e.ax.copy_(engine.upload(stale))
+    D.refresh_x(e, engine.upload(a.acol), engine.upload(x1))
+    torch.cuda.synchronize()
+    assert bitwise_equal(e.ax.cpu().numpy(), H.AXT(m, x1, th=th).ax), seed
