@@ -36,6 +36,9 @@ def parse():
     p.add_argument("--workload", default="powerlaw_1m")
     p.add_argument("--quick", action="store_true", help="headline kernel only (profiling)")
     p.add_argument("--batch-only", action="store_true", help="the batched launch only (profiling)")
+    p.add_argument("--gather-only", type=int, default=0, metavar="K",
+                   help="only the FP64 column-map product with K right-hand sides on the workload's "
+                        "matrix, tile height 1 (profiling)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--batch-count", type=int, default=256,
                    help="matrices in the batched-launch measurement (0 = skip)")
@@ -330,6 +333,17 @@ def main():
     log(f"[bench] workload {args.workload}: n={n} nnz={nnz} rmax={m.stats()['rmax']} "
         f"({time.perf_counter() - t:.1f}s)")
     eng = D.Engine(0)
+    if args.gather_only:  # the column-map product alone (profiling)
+        k = args.gather_only
+        ag = H.AXT(m, x, th=1, want_acol=True)
+        g = D.DeviceAXTGather(eng, ag)
+        X = torch.rand(n * k, dtype=torch.float64, device=eng.torch_device)
+        Y = eng.empty(n * k)
+        w, e = timed(lambda: g.matvec(X, Y, k=k), args.steps, args.warmup, torch)
+        b = M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, k=k)
+        print(json.dumps({"gather": {"workload": args.workload, "k": k, "us": round(e * 1e6, 2),
+                                     "bytes": b, "GBs": round(b / e / 1e9, 1)}}), flush=True)
+        return
     xd = eng.upload(x)
     formats = {}
 
