@@ -47,3 +47,25 @@ def test_driver_kernels_table(gen):
         assert row["et"] > 0, (gen, name)
         assert row["serr"] <= 1e-12, (gen, name, row)
     assert rows["spmv_csr_ordered"]["aerr"] == 0.0  # the oracle's own order
+
+
+@pytest.mark.parametrize("seed,ext", [(1, ".mtx"), (2, ".bin"), (8, ".mtx")])
+def test_driver_on_random_matrix_file(tmp_path, seed, ext):
+    """The driver on a random ragged matrix read from a file (runs of empty
+    rows, one long row, duplicate columns): every kernel within the FP64
+    tolerance, and the statistics file written beside the matrix."""
+    from .test_gpu_random import _case
+    exe = os.path.join(B.LIBDIR, "intspmv_main")
+    assert os.path.exists(exe), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    m = _case(seed)[0]
+    path = tmp_path / f"rand{seed}{ext}"
+    m.write(str(path))
+    print(f"[driver] {path.name} n={m.nrows} nnz={m.nnz}", flush=True)
+    r = subprocess.run([exe, str(path), "--ite", "2", "--x", "uniform"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = _table(r.stdout)
+    assert sorted(rows) == sorted(KERNELS), sorted(rows)
+    for name, row in rows.items():
+        assert row["serr"] <= 1e-12, (seed, name, row)
+    assert (tmp_path / f"rand{seed}.sta").exists()
