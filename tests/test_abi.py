@@ -74,3 +74,23 @@ def test_headers_compile_and_link_as_c(tmp_path):
                         f"-L{lib}", "-lintspmv", "-lintspmv_host", "-L/opt/rocm/lib",
                         "-Wl,--allow-shlib-undefined"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+def test_null_context_is_refused_before_any_hip_call():
+    """Every entry point that takes a context refuses a NULL one (and NULL
+    arrays) with INTSPMV_ERR_ARG before any HIP call, so it runs on a host
+    without a GPU; the destroy calls accept NULL like free()."""
+    from int_spmv_amd import device as D
+    L = D.lib()
+    skip = {"intspmv_create", "intspmv_strerror", "intspmv_device_count", "intspmv_last_hip_error"}
+    checked = 0
+    for name, _, argtypes in D.SIGNATURES:
+        if name in skip or not argtypes:
+            continue
+        args = [None if (t is ctypes.c_void_p or t is ctypes.c_char_p or
+                         (isinstance(t, type) and issubclass(t, ctypes._Pointer))) else t(0)
+                for t in argtypes]
+        st = getattr(L, name)(*args)
+        assert st == (0 if name.endswith("_destroy") else -1), (name, st)
+        checked += 1
+    assert checked >= 30
