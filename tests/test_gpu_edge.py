@@ -105,3 +105,32 @@ def test_every_format_on_degenerate_matrices(engine, name, m):
     for j in range(k3):
         want = O.cf_csr(m.row, m.col, v32, X[:, j].astype(np.float32).astype(np.float64))
         assert np.allclose(Ybh[:, j], want, rtol=1e-6, atol=0), (name, j)
+
+
+def test_bad_arguments_are_refused_before_launch(engine):
+    """Inconsistent sizes and unsupported blockings come back as
+    INTSPMV_ERR_ARG (IntSpmvError, status -1) from the host-side checks, and
+    nothing is written."""
+    import torch
+    m = H.CSR.banded(500, 2, 20, seed=3)
+    n = m.nrows
+    a = H.AXT(m, np.zeros(n), th=1, want_acol=True)
+    g64 = D.DeviceAXTGather(engine, a)
+    X = torch.rand(n * 40, dtype=torch.float64, device=engine.torch_device)
+    Y = _nan(engine, n * 40)
+    bad = [dict(k=0), dict(k=33), dict(k=4, ldx=3), dict(k=4, ldy=3), dict(k=4, kc=3),
+           dict(k=16, kc=16), dict(k=8, kc=32)]
+    for kw in bad:
+        with pytest.raises(D.IntSpmvError) as ei:
+            g64.matvec(X, Y, **kw)
+        assert ei.value.status == -1, kw
+    with pytest.raises(D.IntSpmvError) as ei:
+        g64.matvec(None, Y)
+    assert ei.value.status == -1
+    ent = dict(a=a, X=X[:n * 4], Y=Y[:n * 4], k=4)
+    for kw in (dict(kc=16), dict(kc=3), dict(ldx=3), dict(umax=4), dict(umax=256)):
+        with pytest.raises(D.IntSpmvError) as ei:
+            D.DeviceBatch(engine, [dict(ent, **kw)], D.F64, D.F64)
+        assert ei.value.status == -1, kw
+    torch.cuda.synchronize()
+    assert torch.isnan(Y).all()
