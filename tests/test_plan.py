@@ -3,6 +3,8 @@ row is covered exactly once, every unit exactly once, limits hold, split
 pieces are consecutive and numbered, long blocks precede normal ones."""
 import numpy as np
 import pytest
+from hypothesis import given, settings
+from hypothesis import strategies as st
 
 from int_spmv_amd import device as D
 from int_spmv_amd import host as H
@@ -58,3 +60,19 @@ def test_plan_layout_splits_and_order():
     assert s[:, 0].tolist() == [10, 11, 5000, 29999]
     assert s[:, 2].tolist() == [3, 2, 2, 4]
     assert s[:, 1].tolist() == [0, 3, 5, 7]
+
+
+@settings(max_examples=300, deadline=None, derandomize=True)
+@given(seed=st.integers(0, 2**31 - 1), n=st.integers(1, 400), umax=st.integers(2, 512),
+       rmax=st.integers(1, 256), lu_frac=st.floats(0.0, 0.99), lmax_mul=st.integers(1, 64))
+def test_plan_layout_invariants_random(seed, n, umax, rmax, lu_frac, lmax_mul):
+    """Random row lengths (runs of empty rows, a few rows far above every
+    limit) and random limits: the same invariants."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 40, size=n)
+    lens[rng.random(n) < 0.3] = 0
+    lens[rng.random(n) < 0.02] = rng.integers(200, 5000)
+    uptr = np.zeros(n + 1, np.uint32)
+    uptr[1:] = np.cumsum(lens)
+    lu = max(1, min(umax - 1, int(lu_frac * umax)))
+    _check(uptr, umax, rmax, lu, lu + lmax_mul * 8)
