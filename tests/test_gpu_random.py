@@ -153,14 +153,14 @@ def test_random_batch(engine, seed):
 @pytest.mark.parametrize("seed", range(32))
 def test_random_precision_and_adjoint(engine, seed):
     """The random matrices through the FP32/BF16 embedded kernels (1e-6 /
-    1e-2), the FP32 column map with 20 right-hand sides (a pass of 16 and a
-    pass of 4, 1e-6), the adjoint on the transposed tile map (1e-12 against
+    1e-2), COM in FP32 (1e-6), the BF16 column map (1e-2), the FP32 column
+    map with 20 right-hand sides (a pass of 16 and a pass of 4, 1e-6), the adjoint on the transposed tile map (1e-12 against
     the sequential CSR transpose product; the slot-map read gives the same
     bits) and the value gradient (bit-exact: one product per slot)."""
     import torch
-    m, x, th, _ = _case(seed)
+    m, x, th, bs = _case(seed)
     n = m.nrows
-    print(f"[random precision/adjoint] seed={seed} n={n} nnz={m.nnz} th={th}", flush=True)
+    print(f"[random precision/adjoint] seed={seed} n={n} nnz={m.nnz} th={th} bs={bs}", flush=True)
     dev = engine.torch_device
     yr = O.cf_csr(m.row, m.col, m.val, x)
     a = H.AXT(m, x, th=th)
@@ -168,7 +168,16 @@ def test_random_precision_and_adjoint(engine, seed):
         y = _y(engine, n, np.float32)
         D.DeviceAXT(engine, a, val_type=vt, x_type=D.F32).int_axt(y)
         assert scaled(m, x, yr, _host(y, n)) <= tol, (seed, vt)
+    y = _y(engine, n, np.float32)
+    D.DeviceAXT(engine, H.AXT(m, x, th=1, mode="COM", bs=bs), val_type=D.F32,
+                x_type=D.F32).int_axt(y)
+    assert scaled(m, x, yr, _host(y, n)) <= 1e-6, (seed, "COM F32")
     ag = H.AXT(m, np.zeros(n), th=th, want_acol=True)
+    y = _y(engine, n, np.float32)
+    D.DeviceAXTGather(engine, ag, D.BF16, D.F32).matvec(
+        torch.tensor(x, dtype=torch.float32, device=dev), y)
+    x32 = x.astype(np.float32).astype(np.float64)
+    assert scaled(m, x32, O.cf_csr(m.row, m.col, m.val, x32), _host(y, n)) <= 1e-2, (seed, "BF16")
     g32 = D.DeviceAXTGather(engine, ag, D.F32, D.F32)
     k = 20
     X = np.random.default_rng(seed).uniform(-1, 1, (n, k)).astype(np.float32).astype(np.float64)
