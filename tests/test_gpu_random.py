@@ -109,17 +109,19 @@ def test_random_matrix_all_products(engine, seed):
         assert scaled(m, X[:, j], O.cf_csr(m.row, m.col, m.val, X[:, j]), Yg[:, j]) <= 1e-12, (seed, j)
 
 
-@pytest.mark.parametrize("seed", range(16))
+@pytest.mark.parametrize("seed", range(18))
 def test_random_batch(engine, seed):
     """One batched launch over 1..12 of the random matrices, each with its own
     tile height, 1..32 right-hand sides, a leading dimension of x up to two
-    past k, a drawn RHS blocking kc and units per wave block; FP64 (even
-    seeds, 1e-12) or FP32 storage and sums (odd seeds, 1e-6 against CSR-naive
-    of the FP32-rounded values)."""
+    past k, a drawn RHS blocking kc and units per wave block; by seed: FP64
+    (1e-12), FP32 storage and sums (1e-6 against CSR-naive of the
+    FP32-rounded values) or BF16 values with FP32 x and sums (1e-2)."""
     import torch
     rng = np.random.default_rng(5000 + seed)
-    f64 = seed % 2 == 0
-    vt, dt, tol = (D.F64, torch.float64, 1e-12) if f64 else (D.F32, torch.float32, 1e-6)
+    mode = seed % 3
+    f64 = mode == 0
+    vt, xt, dt, tol = [(D.F64, D.F64, torch.float64, 1e-12), (D.F32, D.F32, torch.float32, 1e-6),
+                       (D.BF16, D.F32, torch.float32, 1e-2)][mode]
     dev = engine.torch_device
     ents = []
     for i in range(int(rng.integers(1, 13))):
@@ -138,13 +140,13 @@ def test_random_batch(engine, seed):
     print(f"[random batch] seed={seed} " + " ".join(
         f"(n={e['m'].nrows} th={e['a'].tileH} k={e['k']} ldx={e['ldx']} kc={e['kc']} umax={e['umax']})"
         for e in ents), flush=True)
-    b = D.DeviceBatch(engine, ents, vt, vt)
+    b = D.DeviceBatch(engine, ents, vt, xt)
     b.launch()
     torch.cuda.synchronize()
     for i, e in enumerate(ents):
         m, k = e["m"], e["k"]
         Y = e["Y"].double().cpu().numpy().reshape(m.nrows, k)
-        vals = m.val if f64 else m.val.astype(np.float32).astype(np.float64)
+        vals = m.val.astype(np.float32).astype(np.float64) if mode == 1 else m.val
         for j in range(k):
             yr = O.cf_csr(m.row, m.col, vals, e["Xh"][:, j])
             assert scaled(m, e["Xh"][:, j], yr, Y[:, j]) <= tol, (seed, i, j)
