@@ -155,7 +155,8 @@ def test_random_batch(engine, seed):
 @pytest.mark.parametrize("seed", range(32))
 def test_random_precision_and_adjoint(engine, seed):
     """The random matrices through the FP32/BF16 embedded kernels (1e-6 /
-    1e-2), COM in FP32 (1e-6), the BF16 column map (1e-2), the FP32 column
+    1e-2), the planar layout with FP64 and FP32 values (1e-12 / 1e-6), COM in
+    FP32 (1e-6), the BF16 column map (1e-2), the FP32 column
     map with 20 right-hand sides (a pass of 16 and a pass of 4, 1e-6), the adjoint on the transposed tile map (1e-12 against
     the sequential CSR transpose product; the slot-map read gives the same
     bits) and the value gradient (bit-exact: one product per slot)."""
@@ -170,6 +171,11 @@ def test_random_precision_and_adjoint(engine, seed):
         y = _y(engine, n, np.float32)
         D.DeviceAXT(engine, a, val_type=vt, x_type=D.F32).int_axt(y)
         assert scaled(m, x, yr, _host(y, n)) <= tol, (seed, vt)
+    # planar value / x arrays with FP64 and FP32 values
+    for vt, dt, tol in ((D.F64, np.float64, 1e-12), (D.F32, np.float32, 1e-6)):
+        y = _y(engine, n, dt)
+        D.DeviceAXT(engine, a, val_type=vt, x_type=vt, layout=D.PLANAR).int_axt(y)
+        assert scaled(m, x, yr, _host(y, n)) <= tol, (seed, vt, "planar")
     y = _y(engine, n, np.float32)
     D.DeviceAXT(engine, H.AXT(m, x, th=1, mode="COM", bs=bs), val_type=D.F32,
                 x_type=D.F32).int_axt(y)
