@@ -29,6 +29,15 @@ def powerlaw_1m(seed: int = 0):
     return m, H.uniform(m.nrows, -1.0, 1.0, seed=seed)
 
 
+def powerlaw_4m(seed: int = 0):
+    """The power-law generator at 2^22 rows and 67M entries (the R-MAT
+    workload's size): how the headline kernel's rate changes with size."""
+    n = 1 << 22
+    m = H.CSR.powerlaw(n=n, target_nnz=67_000_000, s=2.0, maxlen=8192, band_frac=0.8,
+                       band_half=int(0.025 * n), seed=seed)
+    return m, H.uniform(m.nrows, -1.0, 1.0, seed=seed)
+
+
 def stencil27_128(seed: int = 0):
     m = H.CSR.stencil27(128, sigma=0.01, seed=seed)
     return m, H.uniform(m.nrows, -1.0, 1.0, seed=seed)
@@ -57,6 +66,7 @@ def batch_256(seed: int = 0, count: int = 256, nmax: int = 200_000):
 WORKLOADS = {
     "laplace_256": laplace_256,
     "powerlaw_1m": powerlaw_1m,
+    "powerlaw_4m": powerlaw_4m,
     "stencil27_128": stencil27_128,
     "rmat_22": rmat_22,
 }
