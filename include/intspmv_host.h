@@ -139,6 +139,32 @@ int intspmv_axt_build(const intspmv_csr *m, const double *x, uint32_t bs, uint32
 void intspmv_axt_free(intspmv_axt *a);
 /* Rewrites the embedded-x half of ax from acol and a new x (padding stays 0). */
 int intspmv_axt_refresh_x(intspmv_axt *a, const double *x);
+/* Stored values in slot order (slot s = T*tileH*8 + R*8 + C): the value
+ * half of every brick-row of ax, lenAX/2 entries -- the aval array of the
+ * column-map entry points (spmv_axt_gather and the adjoint). */
+int intspmv_axt_value_plane(const intspmv_axt *a, double *vals);
+
+/* The transposed tile map for the adjoint X_bar = A^T Y_bar, built once per
+ * matrix, as getFormatDataAXT (intSpmv_main.c:1275-1381) builds the arrays
+ * int_axt_noc (1443) consumes:
+ *   at     A^T in AXT NOC form with A's tile height, built from the CSR
+ *          transpose with acol/nzslot (values in its value half, x half 0);
+ *   permT  [lenPERM = at.lenAX/2]: permT[s] = the slot of A's layout holding
+ *          the entry that slot s of A^T's layout holds (INTSPMV_ACOL_PAD on
+ *          padding).  spmv_permute_vals(permT) moves A's slot values into
+ *          A^T's order; spmv_axt_gather_permuted reads them through it.
+ * a must be A's NOC layout built with want_acol (for nzslot).  ct is the
+ * build time in seconds. */
+typedef struct {
+    intspmv_axt at;
+    uint32_t lenPERM;
+    uint32_t *permT;
+    double ct;
+} intspmv_axt_tmap;
+int intspmv_axt_transpose_map(const intspmv_csr *m, const intspmv_axt *a,
+                              intspmv_axt_tmap *out);
+void intspmv_axt_tmap_free(intspmv_axt_tmap *t);
+
 /* Lengths only (getArraysLenAXT): fills con/tileN/lenAX/lenSEC/colN. */
 int intspmv_axt_lengths(const intspmv_csr *m, uint32_t thw, uint32_t th, const char *mode,
                         uint32_t *con, uint32_t *tileN, uint32_t *lenAX, uint32_t *lenSEC,

@@ -41,6 +41,7 @@ class AxtSpMV(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vals, x, op: AxtOperator):
         vals, x = vals.contiguous(), x.contiguous()
+        op.eng.follow_current_stream()
         y = torch.empty(op.m.nrows, dtype=op.dtype, device=x.device)
         op.fwd.matvec(x, y, vals=vals)
         ctx.save_for_backward(vals, x)
@@ -52,6 +53,7 @@ class AxtSpMV(torch.autograd.Function):
         vals, x = ctx.saved_tensors
         op = ctx.op
         ybar = ybar.contiguous()
+        op.eng.follow_current_stream()
         gx = gv = None
         if ctx.needs_input_grad[1]:
             gx = torch.empty_like(x)

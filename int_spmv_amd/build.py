@@ -4,6 +4,7 @@ Produces, in-tree (so the libraries travel with the source tree):
   int_spmv_amd/lib/libintspmv_host.so  C host: I/O, stats, builders, generators
   int_spmv_amd/lib/libintspmv.so       gfx950 HIP kernels behind the C ABI
   int_spmv_amd/lib/intspmv_main        C driver (the reference's main sequence)
+  tests/c/adjoint_from_c               C caller of both libraries run by a GPU test
 
 Run ``python -m int_spmv_amd.build`` or call :func:`build_all`.
 """
@@ -74,14 +75,28 @@ def build_driver() -> str:
     return out
 
 
+def build_c_tests() -> str:
+    """C host programs the GPU tests run (tests/c/*.c, linked like any C
+    caller of the two libraries)."""
+    src = os.path.join(ROOT, "tests", "c", "adjoint_from_c.c")
+    out = os.path.join(ROOT, "tests", "c", "adjoint_from_c")
+    if not os.path.exists(src):
+        return ""
+    _run([CC, "-O2", "-std=c11", "-Wall", "-Wextra", f"-I{INCLUDE}", src, "-o", out,
+          f"-L{LIBDIR}", "-lintspmv_host", "-lintspmv", f"-Wl,-rpath,{LIBDIR}",
+          "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lamdhip64", "-lm"])
+    return out
+
+
 def build_all() -> None:
     build_host()
     build_device()
     build_driver()
+    build_c_tests()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"host": build_host, "device": build_device, "driver": build_driver,
-         "all": build_all}[w]()
+         "ctests": build_c_tests, "all": build_all}[w]()

@@ -413,3 +413,72 @@ void intspmv_axt_free(intspmv_axt *a)
     free(a->nzslot);
     memset(a, 0, sizeof(*a));
 }
+
+/* ------------------------------------------------------ transposed map */
+
+int intspmv_axt_value_plane(const intspmv_axt *a, double *vals)
+{
+    if (!a || !vals || (a->lenAX && !a->ax)) return INTSPMV_HOST_EARG;
+    const uint32_t thw = a->tileHW;
+    const int64_t nb = (int64_t)a->lenAX / (2 * thw); /* brick-rows */
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < nb; b++) memcpy(vals + b * thw, a->ax + b * 2 * thw, thw * sizeof(double));
+    return INTSPMV_HOST_OK;
+}
+
+/* A^T's AXT NOC layout is built by the same builder as A's, from the CSR
+ * transpose (entries of a row of A^T in ascending row order of A).  An entry
+ * k' of A^T is entry perm[k'] of A, which sits in A's slot a->nzslot[perm[k']]
+ * and in A^T's slot at.nzslot[k'], so
+ *   permT[at.nzslot[k']] = a->nzslot[perm[k']],
+ * and every slot of A^T that holds no entry is padding. */
+int intspmv_axt_transpose_map(const intspmv_csr *m, const intspmv_axt *a,
+                              intspmv_axt_tmap *out)
+{
+    if (!m || !a || !out) return INTSPMV_HOST_EARG;
+    memset(out, 0, sizeof(*out));
+    if (!mode_is_noc(a->mode) || !a->acol || !a->nzslot || a->nrows != m->nrows ||
+        a->nnz != m->nnz || a->tileHW != INTSPMV_HBRICK_SIZE)
+        return INTSPMV_HOST_EARG;
+    double t0 = intspmv_wtime();
+    intspmv_csr at;
+    uint32_t *perm = (uint32_t *)malloc((m->nnz ? m->nnz : 1) * sizeof(uint32_t));
+    double *zero = (double *)calloc(m->nrows ? m->nrows : 1, sizeof(double));
+    if (!perm || !zero) {
+        free(perm);
+        free(zero);
+        return INTSPMV_HOST_ENOMEM;
+    }
+    int st = intspmv_csr_transpose(m, &at, perm);
+    if (st == 0) {
+        st = intspmv_axt_build(&at, zero, 0, a->tileHW, a->tileH, "NOC", 1, &out->at, NULL);
+        intspmv_csr_free(&at);
+    }
+    free(zero);
+    if (st) {
+        free(perm);
+        memset(out, 0, sizeof(*out));
+        return st;
+    }
+    const size_t ns = (size_t)out->at.lenAX / 2;
+    out->lenPERM = (uint32_t)ns;
+    out->permT = (uint32_t *)malloc((ns ? ns : 1) * sizeof(uint32_t));
+    if (!out->permT) {
+        free(perm);
+        intspmv_axt_tmap_free(out);
+        return INTSPMV_HOST_ENOMEM;
+    }
+    for (size_t s = 0; s < ns; s++) out->permT[s] = INTSPMV_ACOL_PAD;
+    for (uint32_t k = 0; k < m->nnz; k++) out->permT[out->at.nzslot[k]] = a->nzslot[perm[k]];
+    free(perm);
+    out->ct = intspmv_wtime() - t0;
+    return INTSPMV_HOST_OK;
+}
+
+void intspmv_axt_tmap_free(intspmv_axt_tmap *t)
+{
+    if (!t) return;
+    intspmv_axt_free(&t->at);
+    free(t->permT);
+    memset(t, 0, sizeof(*t));
+}

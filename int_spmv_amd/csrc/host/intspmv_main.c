@@ -8,12 +8,11 @@
  *   256, 512) -> "formats' data" and "kernels' results" tables.
  *
  * Differences from the reference's main, each deliberate:
- *   - yr comes from the GPU's ordered CSR kernel, which produces the same bits
- *     as cf_CSR, instead of a CPU loop;
  *   - each kernel is timed as NUM_ITE back-to-back launches between two
  *     stream synchronisations (per-call host timing would measure launch
- *     latency, not the kernel); y is not re-zeroed because every entry point
- *     overwrites y;
+ *     latency, not the kernel); y is filled with NaN once before a kernel's
+ *     launches instead of zeroed before each (every entry point overwrites y,
+ *     and a row a kernel leaves unwritten then shows as an infinite error);
  *   - besides getErrors' global figures the table prints the per-row scaled
  *     error max_i |y_i - yr_i| / sum_j |a_ij x_j|;
  *   - the COM rows are correct results (the reference's COM kernel drops
@@ -55,6 +54,27 @@ static void die(const char *what, int st)
         int st_ = (call);                                                                 \
         if (st_) die(#call, st_);                                                         \
     } while (0)
+
+/* cf_CSR (intSpmv_main.c:382-396), the reference vector yr, on the host as in
+ * the reference's main (1645-1648): per row one fused multiply-add per entry
+ * in CSR order from +0.0 (the reference's compilers contract
+ * aux = aux + val * x). */
+static void cf_csr_host(const intspmv_csr *m, const double *x, double *y)
+{
+    for (uint32_t i = 0; i < m->nrows; i++) {
+        double aux = 0.0;
+        for (uint32_t k = m->row[i]; k < m->row[i + 1]; k++) aux = fma(m->val[k], x[m->col[k]], aux);
+        y[i] = aux;
+    }
+}
+
+static double *nan_host; /* NaNs, as many as the longest output vector */
+
+/* Fills count doubles of a device vector with NaN before a kernel runs. */
+static void poison(void *d, size_t count)
+{
+    CK(intspmv_memcpy_htod(ctx, d, nan_host, count * sizeof(double)));
+}
 
 static void *up(const void *h, size_t bytes)
 {
@@ -218,6 +238,9 @@ int main(int argc, char **argv)
     int nfd = 0;
     void *dy;
     CK(intspmv_malloc(ctx, &dy, ((size_t)n + 8) * sizeof(double)));
+    nan_host = malloc(((size_t)n + 16) * sizeof(double));
+    for (size_t i = 0; i < (size_t)n + 16; i++) nan_host[i] = NAN;
+    cf_csr_host(&m, x, yr);
 
     /* ---- CSR ------------------------------------------------------------- */
     intspmv_csr_format_data(&m, &fds[nfd++]);
@@ -230,13 +253,13 @@ int main(int argc, char **argv)
     c.a2 = drow;
     c.a3 = dx;
     c.y = dy;
-    CK(l_csr_ordered(&c));
-    CK(intspmv_memcpy_dtoh(ctx, yr, dy, (size_t)n * 8)); /* reference vector (cf_CSR bits) */
+    poison(dy, n);
     double et = time_launches(l_csr_ordered, &c);
     finish("spmv_csr_ordered", et, 0, &m, scale, yr, dy, &res[nres++]);
     intspmv_plan *pcsr;
     CK(intspmv_csr_plan(ctx, n, m.row, &pcsr));
     c.plan = pcsr;
+    poison(dy, n);
     et = time_launches(l_csr, &c);
     finish("spmv_csr", et, 0, &m, scale, yr, dy, &res[nres++]);
     intspmv_plan_destroy(pcsr);
@@ -253,6 +276,7 @@ int main(int argc, char **argv)
             ca.a0 = up(a.ax, (size_t)a.lenAX * 8);
             ca.a1 = up(a.brp, (size_t)a.lenBRP * 4);
             ca.y = dy;
+            poison(dy, n);
             et = time_launches(l_axc, &ca);
             finish("spmv_axc", et, 0, &m, scale, yr, dy, &res[nres++]);
             intspmv_free(ctx, (void *)ca.a0);
@@ -281,16 +305,19 @@ int main(int argc, char **argv)
             ck.a4 = NULL;
             ck.a5 = dx;
             ck.y = dres;
+            poison(dres, (size_t)k.chunkNum * 8);
             et = time_launches(l_k1, &ck);
             call_t co = {0};
             co.n = n;
             co.a0 = dperm;
             co.a1 = dres;
             co.y = dy;
+            poison(dy, n);
             double ot = time_launches(l_k1_order, &co);
             finish("spmv_k1", et, ot, &m, scale, yr, dy, &res[nres++]);
             ck.a4 = dperm;
             ck.y = dy;
+            poison(dy, n);
             et = time_launches(l_k1, &ck);
             finish("spmv_k1p", et, 0, &m, scale, yr, dy, &res[nres++]);
             intspmv_free(ctx, dres);
@@ -335,6 +362,7 @@ int main(int argc, char **argv)
         ca.a0 = up(a.ax, (size_t)a.lenAX * 8);
         ca.a1 = up(a.sec, (size_t)a.lenSEC * 4);
         ca.y = dy;
+        poison(dy, n);
         et = time_launches(l_axt, &ca);
         char nm[40];
         if (d.mode == INTSPMV_NOC) snprintf(nm, sizeof nm, "spmv_axt_noc_h%u_hw8", a.tileH);
@@ -376,6 +404,7 @@ int main(int argc, char **argv)
     intspmv_free(ctx, dy);
     intspmv_destroy(ctx);
     free(x);
+    free(nan_host);
     free(scale);
     free(yr);
     intspmv_csr_free(&m);

@@ -145,8 +145,10 @@ def _dptr(t) -> C.c_void_p:
 
 
 class Engine:
-    """Owns an intspmv context on one GPU; launches go to torch's current stream
-    of that device unless :meth:`set_stream` is called."""
+    """Owns an intspmv context on one GPU.  Launches go to the stream last
+    given to :meth:`set_stream`: torch's current stream of the device at the
+    time the Engine is made, until :meth:`follow_current_stream` or
+    :meth:`set_stream` points it elsewhere."""
 
     def __init__(self, device: int = 0):
         torch = _torch()
@@ -167,6 +169,14 @@ class Engine:
     def set_stream(self, stream) -> None:
         self.stream = stream
         self._check(lib().intspmv_set_stream(self.h, _vp(stream.cuda_stream)), "set_stream")
+
+    def follow_current_stream(self) -> None:
+        """Points the context at torch's current stream of the device (if it
+        is not there already), so that launches are ordered with the torch
+        work around them."""
+        cur = _torch().cuda.current_stream(self.torch_device)
+        if cur.cuda_stream != self.stream.cuda_stream:
+            self.set_stream(cur)
 
     def sync(self) -> None:
         self._check(lib().intspmv_sync(self.h), "sync")
@@ -308,10 +318,34 @@ class DeviceK1:
                                       _dptr(x), _dptr(y)), "spmv_k1p")
 
 
+def _f32_round_to_odd(a: np.ndarray) -> np.ndarray:
+    """FP64 -> FP32 rounded to odd: truncated toward zero, lowest mantissa bit
+    set when the conversion was inexact.  Rounding that result once more to a
+    format at least two bits narrower (BF16: 8 bits against FP32's 24) gives
+    the correctly rounded value of the FP64 input (Boldo and Melquiond,
+    "Emulation of FMA and correctly rounded sums: proved algorithms using
+    rounding to odd", IEEE TC 2008)."""
+    d = np.ascontiguousarray(a, dtype=np.float64)
+    with np.errstate(over="ignore", invalid="ignore"):
+        f = d.astype(np.float32)
+        over = np.isfinite(d) & (np.abs(f.astype(np.float64)) > np.abs(d))
+        f = np.where(over, np.nextafter(f, np.float32(0)), f)
+        bits = f.view(np.uint32).copy()
+        inexact = np.isfinite(d) & (f.astype(np.float64) != d)
+    bits[inexact] |= np.uint32(1)
+    return bits.view(np.float32)
+
+
 def bf16_bits(a: np.ndarray) -> np.ndarray:
-    """Round-to-nearest-even FP32 -> BF16 bit patterns (uint16)."""
-    f = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    """Round-to-nearest-even BF16 bit patterns (uint16) of FP32 or FP64
+    values.  FP64 input is rounded once (through FP32 rounded to odd), not
+    FP64 -> FP32 -> BF16 to nearest twice."""
+    a = np.asarray(a)
+    f32 = _f32_round_to_odd(a) if a.dtype == np.float64 else np.ascontiguousarray(a, np.float32)
+    f = f32.view(np.uint32).astype(np.uint64)
+    nan = np.isnan(f32)
     r = ((f + 0x7FFF + ((f >> 16) & 1)) >> 16).astype(np.uint16)
+    r[nan] = ((f[nan] >> 16) | 0x40).astype(np.uint16)  # stays a (quiet) NaN
     return r
 
 
@@ -419,27 +453,19 @@ class DeviceAXTGather:
                                                  _dptr(vbar)), "spmv_axt_grad_vals")
 
 
-def transposed_slot_map(m, a, at_csr, perm, at) -> np.ndarray:
-    """permT[sT] = slot in A's layout of the entry held by slot sT of A^T's
-    layout (PAD on padding)."""
-    pt = np.full(at.lenAX // 2, PAD, np.uint32)
-    pt[at.nzslot] = a.nzslot[perm]
-    return pt
-
-
 class AxtAdjoint:
-    """The transposed tile map of an AXT matrix, built once: the AXT layout of
-    A^T (same tile height) plus the slot permutation from A's slot order.
-    ``adjoint`` computes X_bar = A^T Y_bar with the column-map kernel; the
-    values of A^T follow A's values through ``spmv_permute_vals``."""
+    """The transposed tile map of an AXT matrix, built once on the host by
+    intspmv_axt_transpose_map: the AXT layout of A^T (same tile height) plus
+    the slot permutation from A's slot order.  ``adjoint`` computes
+    X_bar = A^T Y_bar with the column-map kernel; the values of A^T follow
+    A's values through ``spmv_permute_vals``."""
 
     def __init__(self, eng: Engine, m, fwd: DeviceAXTGather):
+        from . import host as H
         self.eng, self.fwd = eng, fwd
-        at_csr, perm = m.transpose()
-        zero = np.zeros(m.nrows)
-        at = H_AXT(at_csr, zero, th=fwd.a.tileH, want_acol=True)
-        self.at = at
-        self.permT = eng.upload(transposed_slot_map(m, fwd.a, at_csr, perm, at))
+        self.tmap = H.AxtTransposeMap(m, fwd.a)
+        at = self.at = self.tmap.at
+        self.permT = eng.upload(self.tmap.permT)
         self.nslots_t = at.lenAX // 2
         self.gT = DeviceAXTGather(eng, at, fwd.val_type, fwd.x_type)
         self.refresh(fwd.aval)

@@ -61,6 +61,10 @@ class _AXT(C.Structure):
                 ("nzslot", u32p)]
 
 
+class _TMAP(C.Structure):
+    _fields_ = [("at", _AXT), ("lenPERM", C.c_uint32), ("permT", u32p), ("ct", C.c_double)]
+
+
 class TuneChoice(C.Structure):
     _fields_ = [("tileH", C.c_uint32), ("kc", C.c_uint32), ("slots", C.c_uint64),
                 ("bytes", C.c_uint64)]
@@ -98,6 +102,11 @@ def lib():
         L.intspmv_axt_free.argtypes = [C.POINTER(_AXT)]
         L.intspmv_axt_free.restype = None
         L.intspmv_axt_refresh_x.argtypes = [C.POINTER(_AXT), f64p]
+        L.intspmv_axt_value_plane.argtypes = [C.POINTER(_AXT), f64p]
+        L.intspmv_axt_transpose_map.argtypes = [C.POINTER(_CSR), C.POINTER(_AXT),
+                                                C.POINTER(_TMAP)]
+        L.intspmv_axt_tmap_free.argtypes = [C.POINTER(_TMAP)]
+        L.intspmv_axt_tmap_free.restype = None
         L.intspmv_fill_uniform.argtypes = [f64p, C.c_size_t, C.c_double, C.c_double,
                                            C.c_uint64]
         L.intspmv_fill_uniform.restype = None
@@ -357,7 +366,18 @@ class AXT:
                                        mode.encode(), int(want_acol), C.byref(self._s),
                                        C.byref(self.fd)), f"axt build {mode} th={th}")
 
+    _owner = None  # set on views of a struct that another object frees
+
+    @classmethod
+    def _borrow(cls, s: _AXT, owner) -> "AXT":
+        """A view of an AXT struct that ``owner`` owns (and frees)."""
+        self = cls.__new__(cls)
+        self._s, self._owner, self.fd = s, owner, FormatData()
+        return self
+
     def __del__(self):
+        if self._owner is not None:
+            return
         if getattr(self, "_s", None) is not None and _lib is not None:
             _lib.intspmv_axt_free(C.byref(self._s))
             self._s = None
@@ -386,8 +406,33 @@ class AXT:
     nzslot = property(lambda self: _view(self._s.nzslot, self._s.nnz, np.uint32))
 
     def value_plane(self) -> np.ndarray:
-        """Stored values in slot order (slot = T*tileH*8 + R*8 + C)."""
-        return np.ascontiguousarray(self.ax.reshape(-1, 2, 8)[:, 0, :]).reshape(-1)
+        """Stored values in slot order (slot = T*tileH*8 + R*8 + C),
+        intspmv_axt_value_plane."""
+        out = np.empty(max(self.lenAX // 2, 1), np.float64)
+        _check(lib().intspmv_axt_value_plane(C.byref(self._s), _ptr(out, C.c_double)),
+               "value plane")
+        return out[: self.lenAX // 2]
+
+
+class AxtTransposeMap:
+    """The transposed tile map of an AXT NOC matrix (intspmv_axt_transpose_map):
+    ``at`` is A^T's AXT NOC layout (A's tile height, with acol) and ``permT``
+    maps each slot of A^T's layout to the slot of A's layout holding the same
+    entry (ACOL_PAD on padding).  ``a`` must be built with want_acol."""
+
+    def __init__(self, m: CSR, a: AXT):
+        self._s = _TMAP()
+        _check(lib().intspmv_axt_transpose_map(C.byref(m._s), C.byref(a._s), C.byref(self._s)),
+               "axt transpose map")
+        self.at = AXT._borrow(self._s.at, self)
+
+    def __del__(self):
+        if getattr(self, "_s", None) is not None and _lib is not None:
+            _lib.intspmv_axt_tmap_free(C.byref(self._s))
+            self._s = None
+
+    permT = property(lambda self: _view(self._s.permT, self._s.lenPERM, np.uint32))
+    ct = property(lambda self: self._s.ct)
 
 
 def tune_model(m: CSR, k: int, sv: int = 8, sx: int = 8) -> list[dict]:

@@ -48,6 +48,8 @@ def lib():
             "orc_get_errors": [_u32, _vp, _vp, _vp, _vp, _vp],
             "orc_scaled_err": [_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
             "orc_csr_transpose_product": [_u32, _vp, _vp, _vp, _vp, _vp],
+            "orc_csr_mm": [_u32, _vp, _vp, _vp, _vp, _u32, _u32, _vp, _u32],
+            "orc_scaled_err_mm": [_u32, _vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _u32, _vp],
             "orc_reduce8": [_vp],
             "orc_int_axc": [_u32, _vp, _vp, _vp],
             "orc_int_k1": [_u32, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -63,6 +65,7 @@ def lib():
             getattr(L, k).argtypes = a
             getattr(L, k).restype = None
         L.orc_scaled_err.restype = C.c_double
+        L.orc_scaled_err_mm.restype = C.c_double
         L.orc_reduce8.restype = C.c_double
         L.orc_bench.restype = C.c_double
         _lib = L
@@ -100,6 +103,32 @@ def scaled_err(row, col, val, x, yref, y):
     e = lib().orc_scaled_err(len(row) - 1, _p(row), _p(col), _p(val), _p(x), _p(yref), _p(y),
                              C.byref(am))
     return e, am.value
+
+
+def csr_mm(row, col, val, X):
+    """cf_CSR of every column of X (n x k): Y (n x k), each column with the
+    bits of cf_csr on that column."""
+    row, col, val = _c(row, np.uint32), _c(col, np.uint32), _c(val, np.float64)
+    X = _c(X, np.float64)
+    if X.ndim == 1:
+        X = X[:, None]
+    n, k = len(row) - 1, X.shape[1]
+    Y = np.zeros((n, k), np.float64)
+    lib().orc_csr_mm(n, _p(row), _p(col), _p(val), _p(X), k, k, _p(Y), k)
+    return Y
+
+
+def scaled_err_mm(row, col, val, X, Yref, Y):
+    """Largest per-entry scaled error over all columns: (err, (row, col))."""
+    row, col, val = _c(row, np.uint32), _c(col, np.uint32), _c(val, np.float64)
+    X, Yref, Y = _c(X, np.float64), _c(Yref, np.float64), _c(Y, np.float64)
+    if X.ndim == 1:
+        X, Yref, Y = X[:, None], Yref[:, None], Y[:, None]
+    n, k = len(row) - 1, X.shape[1]
+    am = C.c_uint64()
+    e = lib().orc_scaled_err_mm(n, _p(row), _p(col), _p(val), _p(X), k, k, _p(Yref), _p(Y), k,
+                                C.byref(am))
+    return e, divmod(am.value, k)
 
 
 def csr_transpose_product(row, col, val, ybar):

@@ -83,6 +83,69 @@ double orc_scaled_err(uint32_t n, const uint32_t *row, const uint32_t *col, cons
     return worst;
 }
 
+/* cf_CSR for k right-hand sides at once: Y[i*ldy + j] is column j's cf_CSR
+ * chain (one fused multiply-add per entry in CSR order from +0.0), so every
+ * column has the bits of orc_cf_csr on that column.  Rows are independent,
+ * so they are spread over the host's threads (checker only; not timed). */
+void orc_csr_mm(uint32_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                const double *X, uint32_t ldx, uint32_t k, double *Y, uint32_t ldy)
+{
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+        const uint32_t i = (uint32_t)ii;
+        double *y = Y + (size_t)i * ldy;
+        for (uint32_t j = 0; j < k; j++) y[j] = 0.0;
+        for (uint32_t e = row[i]; e < row[i + 1]; e++) {
+            const double a = val[e], *x = X + (size_t)col[e] * ldx;
+            for (uint32_t j = 0; j < k; j++) y[j] = fma(a, x[j], y[j]);
+        }
+    }
+}
+
+/* orc_scaled_err over k right-hand sides: the largest
+ * |Y_ij - Yref_ij| / sum_e |a_ie X_col(e),j| over rows i and columns j, with
+ * the same conventions as orc_scaled_err (zero scale: infinite unless equal;
+ * NaN: infinite).  *argmax = i * k + j of the worst entry. */
+double orc_scaled_err_mm(uint32_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                         const double *X, uint32_t ldx, uint32_t k, const double *Yref,
+                         const double *Y, uint32_t ldy, uint64_t *argmax)
+{
+    double worst = 0.0;
+    uint64_t arg = 0;
+#pragma omp parallel
+    {
+        double w = 0.0;
+        uint64_t a = 0;
+        double *s = (double *)calloc(k ? k : 1, sizeof(double));
+#pragma omp for schedule(dynamic, 1024) nowait
+        for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+            const uint32_t i = (uint32_t)ii;
+            for (uint32_t j = 0; j < k; j++) s[j] = 0.0;
+            for (uint32_t e = row[i]; e < row[i + 1]; e++) {
+                const double *x = X + (size_t)col[e] * ldx;
+                for (uint32_t j = 0; j < k; j++) s[j] += fabs(val[e] * x[j]);
+            }
+            for (uint32_t j = 0; j < k; j++) {
+                double d = fabs(Y[(size_t)i * ldy + j] - Yref[(size_t)i * ldy + j]);
+                double e = (s[j] > 0.0) ? d / s[j] : (d == 0.0 ? 0.0 : INFINITY);
+                if (e != e) e = INFINITY;
+                if (e > w) {
+                    w = e;
+                    a = (uint64_t)i * k + j;
+                }
+            }
+        }
+        free(s);
+#pragma omp critical
+        if (w > worst || (w == worst && a < arg && w > 0.0)) {
+            worst = w;
+            arg = a;
+        }
+    }
+    if (argmax) *argmax = arg;
+    return worst;
+}
+
 /* x_bar = A^T y_bar by a sequential scatter in CSR order (the transpose
  * product the adjoint is checked against; not in the reference). */
 void orc_csr_transpose_product(uint32_t n, const uint32_t *row, const uint32_t *col,

@@ -33,6 +33,11 @@ void orc_get_errors(uint32_t len, const double *ar, const double *ac, double *aE
                     double *rErr, uint32_t *pos);
 double orc_scaled_err(uint32_t n, const uint32_t *row, const uint32_t *col, const double *val,
                       const double *x, const double *yref, const double *y, uint32_t *argmax);
+void orc_csr_mm(uint32_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                const double *X, uint32_t ldx, uint32_t k, double *Y, uint32_t ldy);
+double orc_scaled_err_mm(uint32_t n, const uint32_t *row, const uint32_t *col, const double *val,
+                         const double *X, uint32_t ldx, uint32_t k, const double *Yref,
+                         const double *Y, uint32_t ldy, uint64_t *argmax);
 void orc_csr_transpose_product(uint32_t n, const uint32_t *row, const uint32_t *col,
                                const double *val, const double *ybar, double *xbar);
 double orc_reduce8(const double *p);

@@ -232,3 +232,51 @@ def test_generators_deterministic():
     assert np.all((r.val > 0))
     x0 = H.uniform(1000, seed=0)
     assert np.all((x0 >= -1) & (x0 < 1)) and np.array_equal(x0, H.uniform(1000, seed=0))
+
+
+def _slot_rows(a):
+    """Row of every slot of an AXT NOC layout (th == 1: the tile's sec word;
+    th > 1: the sec word of the slot's tile column)."""
+    ns, th = a.lenAX // 2, a.tileH
+    s = np.arange(ns, dtype=np.int64)
+    T, C = s // (8 * th), s % 8
+    return a.sec[T] if th == 1 else a.sec[8 * T + C]
+
+
+@pytest.mark.parametrize("th", [1, 2, 3, 4, 8, 16])
+def test_axt_transpose_map(th):
+    """intspmv_axt_transpose_map: slot s of A^T's layout holds the entry of
+    A's slot permT[s] transposed (its column is that slot's row, its row that
+    slot's column, its value that slot's value); every stored entry of A
+    appears exactly once; padding maps to ACOL_PAD; and A^T's arrays are those
+    the AXT builder makes from the CSR transpose."""
+    from .util import csr_with_empty_rows
+    PAD = 0xFFFFFFFF
+    for m in (csr_with_empty_rows(), H.CSR.powerlaw(n=3000, target_nnz=60000, maxlen=900, seed=4)):
+        a = H.AXT(m, np.zeros(m.nrows), th=th, want_acol=True)
+        t = H.AxtTransposeMap(m, a)
+        at, pt = t.at, t.permT
+        assert len(pt) == at.lenAX // 2 and at.tileH == th and at.nnz == m.nnz
+        real = pt != PAD
+        assert np.array_equal(real, at.acol != PAD)
+        assert np.count_nonzero(real) == m.nnz
+        src = pt[real].astype(np.int64)
+        assert len(np.unique(src)) == m.nnz and np.all(a.acol[src] != PAD)
+        assert np.array_equal(at.acol[real], _slot_rows(a)[src])
+        assert np.array_equal(_slot_rows(at)[real], a.acol[src])
+        assert np.array_equal(at.value_plane()[real], a.value_plane()[src])
+        assert np.all(at.value_plane()[~real] == 0)
+        at_csr, _ = m.transpose()
+        ref = H.AXT(at_csr, np.zeros(m.nrows), th=th, want_acol=True)
+        for f in ("ax", "sec", "con", "acol"):
+            assert np.array_equal(getattr(at, f), getattr(ref, f)), f
+
+
+def test_axt_transpose_map_refuses_bad_input():
+    m = H.CSR.laplace2d(8)
+    with pytest.raises(H.HostError):  # no acol / nzslot
+        H.AxtTransposeMap(m, H.AXT(m, np.zeros(m.nrows), th=1))
+    with pytest.raises(H.HostError):  # COM layouts have no transposed map
+        H.AxtTransposeMap(m, H.AXT(m, np.zeros(m.nrows), th=1, mode="COM", bs=8, want_acol=True))
+    with pytest.raises(H.HostError):  # another matrix
+        H.AxtTransposeMap(H.CSR.laplace2d(9), H.AXT(m, np.zeros(m.nrows), th=1, want_acol=True))

@@ -302,3 +302,20 @@ def test_plan_layouts_are_checked(engine):
                                  D._dptr(g.sec), D._dptr(x), 1, D._dptr(y), 1, 1, 0)
     assert st == -1
     assert e.plan.info()["nblocks"] > 0 and C.sizeof(D.AxtDesc) > 0
+
+
+@pytest.mark.parametrize("th", [1, 4])
+def test_adjoint_from_c(th):
+    """A C host computes x_bar = A^T y_bar through the two C libraries alone
+    (tests/c/adjoint_from_c.c: intspmv_axt_transpose_map, spmv_permute_vals,
+    spmv_axt_gather and spmv_axt_gather_permuted): within 1e-12 of the
+    sequential transpose product, both device paths bit-identical."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c", "adjoint_from_c")
+    assert os.path.exists(exe), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    print(f"[adjoint_from_c] th={th}", flush=True)
+    r = subprocess.run([exe, "40000", "900000", str(th)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "paths_differ 0" in r.stdout, r.stdout

@@ -171,3 +171,100 @@ def test_cpu_baseline_port_matches_oracle(kind, lap256):
     yr = O.cf_csr(m.row, m.col, m.val, x)
     assert O.scaled_err(m.row, m.col, m.val, x, yr, y)[0] <= 1e-12
     assert O.scaled_err(m.row, m.col, m.val, x, ref, y)[0] <= 1e-12
+
+
+def _randn_matrix(n, seed, maxlen):
+    """Random rows of 1..maxlen entries, values N(0,1), x ~ U[-1,1]: products
+    are inexact, so every rounding step of a chain shows in its bits."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(1, maxlen + 1, size=n)
+    row = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    col = np.concatenate([np.sort(rng.choice(n, size=l, replace=False)) for l in lens]).astype(np.uint32)
+    return H.CSR.from_arrays(row, col, rng.standard_normal(len(col))), rng.uniform(-1, 1, n)
+
+
+def _mul_then_add_csr(m, x):
+    """cf_CSR with the product rounded before the add (Python floats)."""
+    y = np.zeros(m.nrows)
+    for i in range(m.nrows):
+        acc = 0.0
+        for k in range(m.row[i], m.row[i + 1]):
+            acc = acc + float(m.val[k]) * float(x[m.col[k]])
+        y[i] = acc
+    return y
+
+
+def test_lanewise_chains_are_fused():
+    # int_k1 / int_k1p keep each row's entries in CSR order in one lane, and
+    # int_axt_noc with a tile height at least the longest row puts each row in
+    # one column: their vtSum chains are then the same sequence of operations
+    # as cf_CSR (padding adds 0*x, which is exact).  cf_CSR is pinned as fused
+    # (test_cf_csr_is_fused), so these chains must be fused too; a
+    # multiply-then-add chain gives other bits on this data (checked below),
+    # so an oracle that rounded the products would fail here.
+    m, x = _randn_matrix(400, 21, 48)
+    yr = O.cf_csr(m.row, m.col, m.val, x)
+    ma = _mul_then_add_csr(m, x)
+    assert np.count_nonzero(ma != yr) > 50, "data not rounding-sensitive"
+    k = H.K1(m)
+    res = O.int_k1(k.chunkNum, k.val, k.col, k.chp, k.nmc, x)
+    assert np.array_equal(O.order_k1(m.nrows, k.permi, res), yr)
+    assert np.array_equal(O.int_k1p(k.chunkNum, m.nrows, k.val, k.col, k.chp, k.nmc, k.permi, x), yr)
+    for th in (48, 64):
+        assert np.array_equal(O.int_axt(H.AXT(m, x, th=th)), yr), th
+
+
+@pytest.mark.parametrize("kind", [0, 2, 3, 6])
+def test_cpu_baseline_port_matches_oracle_more(kind):
+    """The CPU-baseline kernels of cf_CSR, int_k1 (+orderArrayK1), int_k1p and
+    int_axt_com_h1 give the oracle's results (K1: bit for bit; COM: its
+    blocks' atomic order differs from the serial restatement's)."""
+    import ctypes as C
+    m, x = _randn_matrix(3000, 5, 40)
+    L = O.lib()
+    y = np.zeros(m.nrows)
+    keep = []
+
+    def p(a):
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        return C.c_void_p(a.ctypes.data)
+    yr = O.cf_csr(m.row, m.col, m.val, x)
+    if kind == 0:
+        L.orc_bench(0, 1, 2, m.nrows, 0, 0, p(m.row), p(m.col), p(m.val), p(x), None, None, p(y))
+        assert np.array_equal(y, yr)
+    elif kind in (2, 3):
+        k = H.K1(m)
+        L.orc_bench(kind, 3, 2, m.nrows, k.chunkNum, 0, p(k.val), p(k.col), p(k.chp), p(k.nmc),
+                    p(x), p(k.permi), p(y))
+        assert np.array_equal(y, yr)
+    else:
+        a = H.AXT(m, x, th=1, mode="COM", bs=128)
+        bn = (a.tileN * a.tileHW + 127) // 128
+        prm = np.array([128, a.log, a.tileHW, a.lenAX, a.lenSEC], np.uint32)
+        L.orc_bench(6, 3, 2, m.nrows, bn, 1, p(a.ax), p(a.sec), p(prm), None, None, None, p(y))
+        ref = O.int_axt_com_h1(a)
+        assert O.scaled_err(m.row, m.col, m.val, x, ref, y)[0] <= 1e-12
+        # the reference's COM drops row 0 (its head word is 0): everything else
+        # is CSR-naive to rounding
+        keep_rows = np.arange(1, m.nrows)
+        assert np.max(np.abs(y - yr)[keep_rows]) <= 1e-12 * np.max(np.abs(yr))
+
+
+def test_multi_rhs_checker_equals_single_column():
+    """orc_csr_mm gives every column cf_CSR's bits; orc_scaled_err_mm is the
+    largest of orc_scaled_err over the columns, at the same position."""
+    m, _ = _randn_matrix(700, 31, 60)
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-1, 1, (m.nrows, 5))
+    Y = O.csr_mm(m.row, m.col, m.val, X)
+    for j in range(5):
+        assert np.array_equal(Y[:, j], O.cf_csr(m.row, m.col, m.val, X[:, j]))
+    Yp = Y.copy()
+    Yp[123, 3] += 1e-9
+    Yp[7, 1] += 1e-12
+    e, (i, j) = O.scaled_err_mm(m.row, m.col, m.val, X, Y, Yp)
+    per = [O.scaled_err(m.row, m.col, m.val, X[:, c], Y[:, c], Yp[:, c]) for c in range(5)]
+    assert (i, j) == (123, 3) and e == per[3][0] == max(p[0] for p in per) and per[3][1] == 123
+    Yp[0, 0] = np.nan
+    assert O.scaled_err_mm(m.row, m.col, m.val, X, Y, Yp)[0] == np.inf
