@@ -7,8 +7,12 @@ the reference's int_axt_noc_h1, intSpmv_main.c:1385) on the power-law
 workload: n = 2^20, ~30M entries, Zipf(2) row lengths up to 8192.  GB/s are
 algorithmic bytes (int_spmv_amd/metrics.py) per step time.  The other formats,
 tile heights and storage precisions are timed beside it and reported under
-"formats".  The CPU baseline is the OpenMP restatement of the reference's
-kernel on the host's cores (the reference itself needs <mkl.h> to build).
+"formats"; every timed product is then checked against the FP64 oracle
+(per-row scaled error, as the reference's getErrors after each kernel), and
+the 256-matrix batch against cf_CSR of its FP32-rounded inputs.  The CPU
+baseline is the AVX-512 + OpenMP restatement of the reference's kernels
+(oracle/cpu_avx512.c) on the host's cores; the reference itself needs
+<mkl.h> to build.
 
 Usage: python bench.py [--steps K] [--warmup W] [--workload NAME] [--quick]
 """
@@ -52,10 +56,25 @@ def parse():
 
 
 def cores() -> int:
+    """Threads the CPU baseline runs on: OMP_NUM_THREADS, else the cores this
+    process may use (its affinity mask)."""
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit():
         return int(env)
     return len(os.sched_getaffinity(0))
+
+
+def physical_cores():
+    """Physical cores of the host (distinct core/socket pairs in lscpu -p),
+    reported beside the threads the baseline uses."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=10).stdout
+        pairs = {line.strip() for line in out.splitlines() if line and not line.startswith("#")}
+        return len(pairs) or None
+    except (OSError, subprocess.SubprocessError):
+        return None
 
 
 def timed(fn, steps, warmup, torch):
@@ -122,8 +141,9 @@ def bench_batch(eng, count, torch, log, tune="model"):
             c = dict(H.tune_model(m, k, 4, 4)[0], umax=128)
         chosen[(c["tileH"], c["kc"], c["umax"])] += 1
         a = H.AXT(m, np.zeros(m.nrows), th=c["tileH"], want_acol=True)
-        X = torch.rand(m.nrows * k, dtype=torch.float32, device=eng.torch_device)
-        Y = torch.empty(m.nrows * k, dtype=torch.float32, device=eng.torch_device)
+        X = torch.rand(m.nrows * k, dtype=torch.float32, device=eng.torch_device) * 2 - 1
+        Y = torch.full((m.nrows * k,), float("nan"), dtype=torch.float32,
+                       device=eng.torch_device)
         ents.append(dict(a=a, X=X, Y=Y, k=k, kc=c["kc"], umax=c["umax"]))
         flops += M.flops(m.nnz, k)
         bytes_ += M.bytes_axt_gather(m.nrows, a.lenAX, a.lenSEC, k=k, sv=4, sx=4, sy=4)
@@ -153,7 +173,29 @@ def bench_batch(eng, count, torch, log, tune="model"):
          "tuner": tune,
          "choices": {f"th{t}_kc{kc}_u{u}": c for (t, kc, u), c in sorted(chosen.items())}}
     log(f"[bench] BATCH: {r}")
+    r["_check"] = [(m, e["X"], e["Y"], e["k"]) for (m, _), e in zip(mats, ents)]
     return r
+
+
+def check_batch(items):
+    """Every result column of every batch matrix against cf_CSR of its
+    FP32-rounded values and x (the accumulation error of FP32 sums)."""
+    import numpy as np
+
+    import oracle as O
+    worst, where, cols = 0.0, None, 0
+    for i, (m, X, Y, k) in enumerate(items):
+        v32 = m.val.astype(np.float32).astype(np.float64)
+        Xh = X.double().cpu().numpy().reshape(m.nrows, k)
+        Yr = O.csr_mm(m.row, m.col, v32, Xh)
+        e, pos = O.scaled_err_mm(m.row, m.col, v32, Xh, Yr,
+                                 Y.double().cpu().numpy().reshape(m.nrows, k))
+        cols += k
+        if not e <= worst:
+            worst, where = e, [i, int(pos[0]), int(pos[1])]
+    log(f"[bench] batch check: {len(items)} matrices, {cols} columns, max scaled err {worst:.2e}")
+    return {"max_scaled_err_vs_fp32_oracle": worst, "worst_at_matrix_row_col": where,
+            "checked_matrices": len(items), "checked_columns": cols}
 
 
 def rocsparse_leg(dc, xd, y_csr, n, nnz):
@@ -180,10 +222,57 @@ def rocsparse_leg(dc, xd, y_csr, n, nnz):
     return out
 
 
-def cpu_leg(m, x, a1, hosts, formats, y1, head_bytes, workload, quick):
-    """Errors of the GPU results against the FP64 oracle, and the CPU baseline:
-    the oracle's OpenMP restatements of the reference kernels (and MKL's CSR
-    kernels when libmkl_rt is present) timed on this host's cores."""
+def check_results(m, x, checks, formats):
+    """Every timed product against the FP64 oracle, as the reference's
+    getErrors after each kernel (intSpmv_main.c:1507-1510): per-row scaled
+    error max |y - y_ref| / sum |a x| (all right-hand sides; the adjoint
+    against the sequential transpose product; the value gradient bit for
+    bit), held to 1e-12 (FP64), 1e-6 (FP32 storage) or 1e-2 (BF16).
+    Returns the names over their tolerance."""
+    import numpy as np
+
+    import oracle as O
+    n = m.nrows
+    yref = O.cf_csr(m.row, m.col, m.val, x)
+    at_csr = None
+    over = []
+    for name, c in checks.items():
+        tol, kind, y = c.get("tol", 1e-12), c["kind"], c["y"]
+        f = formats[name]
+        if kind == "y":
+            e = O.scaled_err(m.row, m.col, m.val, x, yref, y)[0]
+            if c.get("bitwise"):
+                f["bitwise_vs_cf_csr"] = bool(np.array_equal(y, yref))
+                if not f["bitwise_vs_cf_csr"]:
+                    over.append(name)
+        elif kind == "mm":
+            X = c["X"]
+            Yr = O.csr_mm(m.row, m.col, m.val, X)
+            e = O.scaled_err_mm(m.row, m.col, m.val, X, Yr, y.reshape(n, X.shape[1]))[0]
+        elif kind == "adj":
+            if at_csr is None:
+                at_csr, _ = m.transpose()
+            ref = O.csr_transpose_product(m.row, m.col, m.val, c["ybar"])
+            e = O.scaled_err(at_csr.row, at_csr.col, at_csr.val, c["ybar"], ref, y)[0]
+        else:  # value gradient: ybar[row] * x[col] on the slots, bit for bit
+            rows = np.repeat(np.arange(n), np.diff(m.row.astype(np.int64)))
+            want = np.zeros(len(y))
+            want[c["nzslot"]] = c["ybar"][rows] * x[m.col]
+            e = 0.0 if np.array_equal(y, want) else float("inf")
+        f["max_scaled_err_vs_fp64_oracle"] = e
+        f["tol"] = tol
+        if not e <= tol:
+            over.append(name)
+    if "AXT_NOC_H1_F64" in checks:  # the reference's own figures for the headline
+        a_err, r_err, pos = O.get_errors(yref, checks["AXT_NOC_H1_F64"]["y"])
+        formats["AXT_NOC_H1_F64"]["getErrors"] = {"aErr": a_err, "rErr": r_err, "rowInd": pos}
+    return over
+
+
+def cpu_leg(m, x, a1, hosts, head_bytes, workload, quick):
+    """The CPU baseline: the oracle's OpenMP restatements of the reference
+    kernels (and MKL's CSR kernels when libmkl_rt is present) timed on this
+    host's cores."""
     import ctypes as C
 
     import numpy as np
@@ -196,34 +285,31 @@ def cpu_leg(m, x, a1, hosts, formats, y1, head_bytes, workload, quick):
 
     def serr(y):
         return O.scaled_err(m.row, m.col, m.val, x, yref, y)[0]
-    yh = y1[:n].cpu().numpy()
-    formats["AXT_NOC_H1_F64"]["max_scaled_err_vs_csr"] = serr(yh)
-    a_err, r_err, pos = O.get_errors(yref, yh)  # the reference's getErrors (404-425)
-    formats["AXT_NOC_H1_F64"]["getErrors"] = {"aErr": a_err, "rErr": r_err, "rowInd": pos}
-    for nm in ("AXT_NOC_H1_F32", "AXT_NOC_H1_BF16", "AXT_NOC_H4_F32", "AXT_NOC_H4_BF16"):
-        if nm in formats and "_y" in formats[nm]:
-            yy = formats[nm].pop("_y")[:n].double().cpu().numpy()
-            formats[nm]["max_scaled_err_vs_fp64"] = serr(yy)
     threads = cores()
-    L = O.lib()
+    L = O.lib_avx512()
+    if L is None:  # no scalar stand-in for the reference's AVX-512 kernels
+        return {"value": None, "unit": "GB/s", "cores": threads, "kind": "port",
+                "sample": "n/a: the host CPU has no AVX-512 (the reference's kernels are "
+                          "AVX-512 intrinsics)", "host_physical_cores": physical_cores()}
 
     def run(kind, thr, budget, len_, th, arrs):
         keep = [np.ascontiguousarray(a) for a in arrs]
         ptrs = [C.c_void_p(a.ctypes.data) for a in keep] + [None] * (6 - len(keep))
         y = np.zeros(n)
-        t1 = L.orc_bench(kind, thr, 1, n, len_, th, *ptrs, C.c_void_p(y.ctypes.data))
+        t1 = L.orc_bench_avx512(kind, thr, 1, n, len_, th, *ptrs, C.c_void_p(y.ctypes.data))
         if t1 < 0:
-            raise RuntimeError(f"orc_bench kind {kind}")
+            raise RuntimeError(f"orc_bench_avx512 kind {kind}")
         iters = int(max(1, min(250, budget / max(t1, 1e-6))))
-        t = L.orc_bench(kind, thr, iters, n, len_, th, *ptrs, C.c_void_p(y.ctypes.data))
+        t = L.orc_bench_avx512(kind, thr, iters, n, len_, th, *ptrs, C.c_void_p(y.ctypes.data))
         return t, iters, y
 
     t, iters, ycpu = run(4, threads, 10.0, a1.tileN, 1, [a1.ax, a1.sec])
     cpu = {"value": round(head_bytes / t / 1e9, 2), "unit": "GB/s", "cores": threads,
            "kind": "port",
-           "sample": f"int_axt_noc_h1 OpenMP restatement (dynamic schedule, atomic y), "
-                     f"full {workload} matrix, {iters} calls, y zeroed untimed; "
-                     f"{t * 1e3:.2f} ms/call",
+           "sample": f"int_axt_noc_h1 AVX-512 + OpenMP restatement (oracle/cpu_avx512.c: "
+                     f"dynamic schedule, atomic y), full {workload} matrix, {iters} calls, "
+                     f"y zeroed untimed; {t * 1e3:.2f} ms/call",
+           "host_physical_cores": physical_cores(),
            "max_scaled_err_vs_csr": serr(ycpu)}
     log(f"[bench] cpu baseline: {cpu}")
     if quick:
@@ -279,6 +365,11 @@ def cpu_leg(m, x, a1, hosts, formats, y1, head_bytes, workload, quick):
         others["mkl"] = "libmkl_rt not found"
     cpu["formats"] = others
     return cpu
+
+
+def formats_checked(formats):
+    return [k for k, v in formats.items()
+            if isinstance(v, dict) and "max_scaled_err_vs_fp64_oracle" in v]
 
 
 def read_traffic(name: str):
@@ -346,6 +437,19 @@ def main():
         return
     xd = eng.upload(x)
     formats = {}
+    # results kept for the oracle checks of the CPU leg: name -> (kind, data)
+    checks = {}
+
+    def nanv(count, dt=np.float64):
+        """An output vector filled with NaN, so an unwritten entry fails its check."""
+        t = eng.empty(count, dt)
+        t.fill_(float("nan"))
+        return t
+
+    def keep(name, kind, t, count, **kw):
+        """Host copy of a product's output (taken after its timing)."""
+        torch.cuda.synchronize()
+        checks[name] = dict(kind=kind, y=t.reshape(-1)[:count].double().cpu().numpy(), **kw)
 
     def record(name, bytes_, wall, ev, extra=None):
         r = {"us": round(ev * 1e6, 2), "GBs": round(bytes_ / ev / 1e9, 1),
@@ -360,11 +464,12 @@ def main():
     # ---- headline: AXT NOC th=1, FP64, reference layout ----------------------
     a1 = H.AXT(m, x, th=1)
     d1 = D.DeviceAXT(eng, a1)
-    y1 = eng.empty(n)
+    y1 = nanv(n)
     head_bytes = M.bytes_axt(n, a1.lenAX, a1.lenSEC)
     wall, ev = timed(lambda: d1.int_axt(y1), args.steps, args.warmup, torch)
     record("AXT_NOC_H1_F64", head_bytes, wall, ev,
            {"beta": round(a1.fd.beta, 4), "plan": d1.plan.info()})
+    keep("AXT_NOC_H1_F64", "y", y1, n)
     head_wall, head_ev = wall, ev
     hosts = {}
     if args.quick:  # headline launches only (profiling)
@@ -389,10 +494,11 @@ def main():
         for th in (2, 4, 8, 16, 32, 48):
             a = H.AXT(m, x, th=th)
             d = D.DeviceAXT(eng, a)
-            y = eng.empty(n)
+            y = nanv(n)
             w, e = timed(lambda: d.int_axt(y), 20, 3, torch)
             record(f"AXT_NOC_H{th}_F64", M.bytes_axt(n, a.lenAX, a.lenSEC), w, e,
                    {"beta": round(a.fd.beta, 4)})
+            keep(f"AXT_NOC_H{th}_F64", "y", y, n)
             if th == 4:
                 hosts["AXT_H4"] = a
                 h4_ev = e
@@ -400,10 +506,11 @@ def main():
         for bs in (128, 256, 512):
             a = H.AXT(m, x, th=1, mode="COM", bs=bs)
             d = D.DeviceAXT(eng, a)
-            y = eng.empty(n)
+            y = nanv(n)
             w, e = timed(lambda: d.int_axt(y), 20, 3, torch)
             record(f"AXT_COM_BS{bs}_F64", M.bytes_axt(n, a.lenAX, a.lenSEC), w, e,
                    {"beta": round(a.fd.beta, 4)})
+            keep(f"AXT_COM_BS{bs}_F64", "y", y, n)
             del d, a, y
         a4 = hosts["AXT_H4"]
         h4_bytes = M.bytes_axt(n, a4.lenAX, a4.lenSEC)
@@ -411,20 +518,22 @@ def main():
             for vt, xt, prec, sv, sx in ((D.F32, D.F32, "F32", 4, 4), (D.BF16, D.F32, "BF16", 2, 4)):
                 nm = f"AXT_NOC_H{th}_{prec}"
                 d = D.DeviceAXT(eng, ah, val_type=vt, x_type=xt)
-                y = eng.empty(n, np.float32)
+                y = nanv(n, np.float32)
                 w, e = timed(lambda: d.int_axt(y), 20, 3, torch)
                 own = M.bytes_axt(n, ah.lenAX, ah.lenSEC, sv=sv, sx=sx, sy=4)
                 record(nm, own, w, e, {"GBs_eq_fp64": round(fb / e / 1e9, 1),
                                        "vs_fp64": round(fe / e, 2)})
-                formats[nm]["_y"] = y
+                keep(nm, "y", y, n, tol=1e-6 if prec == "F32" else 1e-2)
                 del d
         # column-map form: 1 and 8 right-hand sides, adjoint on the transposed map
         ag = H.AXT(m, x, th=1, want_acol=True)
         g = D.DeviceAXTGather(eng, ag)
-        y = eng.empty(n)
+        y = nanv(n)
         w, e = timed(lambda: g.matvec(xd, y), 20, 3, torch)
         fwd_g = e
-        record("AXT_GATHER_H1_F64_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC), w, e)
+        record("AXT_GATHER_H1_F64_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC), w, e,
+               {"vs_embedded": round(e / head_ev, 3)})
+        keep("AXT_GATHER_H1_F64_K1", "y", y, n)
         # conversion: a new x into the embedded AXT arrays, on the device
         # (spmv_axt_refresh_x) versus the host builder (getFormatDataAXT)
         t0 = time.perf_counter()
@@ -437,65 +546,80 @@ def main():
         log(f"[bench] AXT_X_REFRESH_H1: {formats['AXT_X_REFRESH_H1']}")
         del dacol
         k8 = 8
-        X8 = torch.rand(n * k8, dtype=torch.float64, device=eng.torch_device)
-        Y8 = eng.empty(n * k8)
+        X8 = torch.rand(n * k8, dtype=torch.float64, device=eng.torch_device) * 2 - 1
+        Y8 = nanv(n * k8)
         w, e = timed(lambda: g.matvec(X8, Y8, k=k8), 10, 2, torch)
         record("AXT_GATHER_H1_F64_K8", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, k=k8), w, e,
                {"GFLOPs": round(M.flops(nnz, k8) / e / 1e9, 1)})
+        keep("AXT_GATHER_H1_F64_K8", "mm", Y8, n * k8, X=X8.cpu().numpy().reshape(n, k8))
         adj = D.AxtAdjoint(eng, m, g)
         yb = torch.randn(n, dtype=torch.float64, device=eng.torch_device)
-        xb = eng.empty(n)
+        xb = nanv(n)
         w, e = timed(lambda: adj.adjoint(yb, xb), 20, 3, torch)
         at = adj.at
         record("AXT_ADJOINT_H1_F64", M.bytes_axt_gather(n, at.lenAX, at.lenSEC), w, e,
                {"vs_forward_gather": round(e / fwd_g, 3), "vs_forward_embedded": round(e / head_ev, 3)})
+        ybh = yb.cpu().numpy()
+        keep("AXT_ADJOINT_H1_F64", "adj", xb, n, ybar=ybh)
         # right after a value update (a training step): A's values read through
         # the slot map, or first copied into Aᵀ's order (a permutation)
+        xb.fill_(float("nan"))
         w, e = timed(lambda: adj.adjoint(yb, xb, vals=g.aval), 20, 3, torch)
         record("AXT_ADJOINT_PERMUTED_H1_F64",
                M.bytes_axt_gather(n, at.lenAX, at.lenSEC) + (at.lenAX // 2) * 4, w, e,
-               {"vs_forward_gather": round(e / fwd_g, 3)})
+               {"vs_forward_gather": round(e / fwd_g, 3),
+                "vs_forward_embedded": round(e / head_ev, 3)})
+        keep("AXT_ADJOINT_PERMUTED_H1_F64", "adj", xb, n, ybar=ybh)
         w, e = timed(lambda: adj.refresh(g.aval), 20, 3, torch)
         record("AXT_ADJOINT_VALUES_H1_F64", (at.lenAX // 2) * (4 + 8 + 8), w, e,
                {"vs_forward_gather": round(e / fwd_g, 3)})
         # which of the two the autograd backward takes on this matrix
         formats["AXT_ADJOINT_VALUES_PATH_H1_F64"] = adj.tune_values_path()
         log(f"[bench] AXT_ADJOINT_VALUES_PATH_H1_F64: {formats['AXT_ADJOINT_VALUES_PATH_H1_F64']}")
-        vb = eng.empty(ag.lenAX // 2)
+        vb = nanv(ag.lenAX // 2)
         w, e = timed(lambda: g.grad_vals(xd, yb, vb), 20, 3, torch)
         record("AXT_GRAD_VALS_H1_F64", (ag.lenAX // 2) * (4 + 8) + ag.lenSEC * 4 + 2 * n * 8, w, e,
                {"vs_forward_gather": round(e / fwd_g, 3)})
+        keep("AXT_GRAD_VALS_H1_F64", "gv", vb, ag.lenAX // 2, ybar=ybh,
+             nzslot=ag.nzslot.copy())
         del g, adj, X8, Y8, vb
         # the same in FP32 storage (values and x FP32, FP32 sums)
         g32 = D.DeviceAXTGather(eng, ag, D.F32, D.F32)
         x32 = xd.float()
-        y32 = torch.empty(n, dtype=torch.float32, device=eng.torch_device)
+        y32 = nanv(n, np.float32)
         w, e = timed(lambda: g32.matvec(x32, y32), 20, 3, torch)
         fwd32 = e
         record("AXT_GATHER_H1_F32_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, sv=4, sx=4, sy=4),
                w, e)
+        keep("AXT_GATHER_H1_F32_K1", "y", y32, n, tol=1e-6)
         k32 = 32  # 16 right-hand sides per pass
-        X32 = torch.rand(n * k32, dtype=torch.float32, device=eng.torch_device)
-        Y32 = torch.empty(n * k32, dtype=torch.float32, device=eng.torch_device)
+        X32 = torch.rand(n * k32, dtype=torch.float32, device=eng.torch_device) * 2 - 1
+        Y32 = nanv(n * k32, np.float32)
         w, e = timed(lambda: g32.matvec(X32, Y32, k=k32), 10, 2, torch)
         record("AXT_GATHER_H1_F32_K32",
                M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, k=k32, sv=4, sx=4, sy=4), w, e,
                {"GFLOPs": round(M.flops(nnz, k32) / e / 1e9, 1)})
+        keep("AXT_GATHER_H1_F32_K32", "mm", Y32, n * k32,
+             X=X32.double().cpu().numpy().reshape(n, k32), tol=1e-6)
         del X32, Y32
         adj32 = D.AxtAdjoint(eng, m, g32)
         yb32 = yb.float()
-        xb32 = torch.empty(n, dtype=torch.float32, device=eng.torch_device)
+        xb32 = nanv(n, np.float32)
         w, e = timed(lambda: adj32.adjoint(yb32, xb32), 20, 3, torch)
         at = adj32.at
         record("AXT_ADJOINT_H1_F32", M.bytes_axt_gather(n, at.lenAX, at.lenSEC, sv=4, sx=4, sy=4),
                w, e, {"vs_forward_gather": round(e / fwd32, 3)})
+        keep("AXT_ADJOINT_H1_F32", "adj", xb32, n, ybar=yb32.double().cpu().numpy(), tol=1e-6)
         del g32, adj32, ag, x32, y32, yb32, xb32
         dc = D.DeviceCSR(eng, m)
-        y = eng.empty(n)
+        y = nanv(n)
         w, e = timed(lambda: dc.spmv(xd, y), 20, 3, torch)
         record("CSR_F64", M.bytes_csr(n, nnz), w, e)
+        keep("CSR_F64", "y", y, n)
+        y.fill_(float("nan"))
         w, e = timed(lambda: dc.cf_csr(xd, y), 3, 1, torch)
         record("CSR_ORDERED_F64", M.bytes_csr(n, nnz), w, e)
+        keep("CSR_ORDERED_F64", "y", y, n, bitwise=True)
         rs = rocsparse_leg(dc, xd, y, n, nnz)
         if rs is None:
             formats["ROCSPARSE_CSR_F64"] = "librocsparse not found"
@@ -507,15 +631,19 @@ def main():
                        {"rel_err_vs_csr": rel})
         c = H.AXC(m, x)
         dcc = D.DeviceAXC(eng, c)
+        y.fill_(float("nan"))
         w, e = timed(lambda: dcc.int_axc(y), 20, 3, torch)
         record("AXC_F64", M.bytes_axc(n, c.lenAX), w, e)
+        keep("AXC_F64", "y", y, n)
         del dcc
         hosts["AXC"] = c
         t0 = time.perf_counter()
         k = H.K1(m)
         host_build = time.perf_counter() - t0
         dk = D.DeviceK1(eng, k)
+        y.fill_(float("nan"))
         w, e = timed(lambda: dk.int_k1p(xd, y), 20, 3, torch)
+        keep("K1P_F64", "y", y, n)
         # the K1 conversion on the device from the device CSR (intspmv_k1_build_device)
         D.DeviceK1.from_device_csr(eng, dc)
         torch.cuda.synchronize()
@@ -534,15 +662,24 @@ def main():
         batch = bench_batch(eng, args.batch_count, torch, log, args.batch_tune)
 
     # ---- CPU baseline + correctness (oracle as the checker) -------------------
-    cpu = None
+    cpu, over = None, None
     if not args.no_cpu:
-        cpu = cpu_leg(m, x, a1, hosts, formats, y1, head_bytes, args.workload, args.quick)
-    for f in formats.values():
-        if isinstance(f, dict):
-            f.pop("_y", None)
+        over = check_results(m, x, checks, formats)
+        del checks
+        if batch is not None:
+            batch.update(check_batch(batch.pop("_check")))
+            if not batch["max_scaled_err_vs_fp32_oracle"] <= 1e-6:
+                over.append("batch")
+        if over:
+            log(f"[bench] OVER TOLERANCE: {over}")
+        cpu = cpu_leg(m, x, a1, hosts, head_bytes, args.workload, args.quick)
+    elif batch is not None:
+        batch.pop("_check")
 
     achieved = head_bytes / head_ev / 1e9
-    traffic = read_traffic(f"axt_h1_f64_{args.workload}_traffic.txt")
+    traffic_file = f"profiles/axt_h1_f64_{args.workload}_traffic.txt"
+    traffic = read_traffic(os.path.basename(traffic_file))
+    cold_us = formats["AXT_NOC_H1_F64"].get("us_cold_median")
     out = {
         "metric": "SpMV HBM GB/s (% of 8 TB/s) + GFLOP/s: AXT vs CSR, FP64 vs FP32/BF16, 1 GPU",
         "value": round(head_bytes / head_wall / 1e9, 1),
@@ -564,7 +701,19 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": M.HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / M.HBM_PEAK_GBS, 4),
                      "traffic": traffic,
+                     "traffic_source": (f"{traffic_file}: rocprofv3 --pmc FETCH_SIZE, then "
+                                        f"WRITE_SIZE, per dispatch of the same kernel on the same "
+                                        f"workload, scaled by a 512 MiB device copy's counts; "
+                                        f"not measured in this run") if traffic else None,
+                     "traffic_vs_algorithmic": round(traffic / head_bytes, 4) if traffic else None,
+                     "bytes_per_launch": int(head_bytes),
+                     "us_per_launch": round(head_ev * 1e6, 2),
+                     "cold_us_median": cold_us,
+                     "cold_frac": (round(head_bytes / (cold_us * 1e-6) / 1e9 / M.HBM_PEAK_GBS, 4)
+                                   if cold_us else None),
                      "kernel": "axt_h1_kernel<double,double,double,false> (+ split fix-up)"},
+        "errors": ({"checked": len(formats_checked(formats)), "over_tolerance": over}
+                   if over is not None else None),
         "cpu_baseline": cpu,
         "formats": formats,
         "batch": batch,

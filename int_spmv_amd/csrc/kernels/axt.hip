@@ -572,7 +572,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
     const BlockDesc d = blocks[bid];
     const uint32_t nu = d.uend - d.ubeg;
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
-    AT acc = 0; // long rows: lane-strided fma partials
+    // long rows: lane-strided partials; FP64 an fma chain, FP32 a
+    // compensated (Neumaier) sum of the products
+    AT acc = 0;
+    LaneSum<AT> cacc;
     // full-line mapping: 8 whole tiles per load, lane part p = l%8; values of
     // slots 2p, 2p+1 (p < 4) meet their x through a DPP row shift
     const uint32_t p = lane & 7, Cq = 2 * (p & 3);
@@ -594,8 +597,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
             const bool ok0 = p < 4 && Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
             const bool ok1 = p < 4 && Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
             if (d.k != 0) {
-                if (ok0) acc = fma_acc<AT>((AT)v0[u], x0, acc);
-                if (ok1) acc = fma_acc<AT>((AT)v1[u], x1, acc);
+                if constexpr (sizeof(AT) == 4) {
+                    if (ok0) cacc.add((AT)v0[u] * x0);
+                    if (ok1) cacc.add((AT)v1[u] * x1);
+                } else {
+                    if (ok0) acc = fma_acc<AT>((AT)v0[u], x0, acc);
+                    if (ok1) acc = fma_acc<AT>((AT)v1[u], x1, acc);
+                }
                 continue;
             }
             if (ok0) {
@@ -611,6 +619,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
         }
     }
     if (d.k != 0) {
+        if constexpr (sizeof(AT) == 4) acc = cacc.value();
         finish_long<AT>(wave_sum<AT>(acc), d, bid, y, nrows, partial);
         return;
     }

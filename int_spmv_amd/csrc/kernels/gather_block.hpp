@@ -123,7 +123,8 @@ __device__ __forceinline__ void gather_x(const XT *x, __amdgpu_buffer_rsrc_t xr,
 // One pass over KC right-hand sides.  th == 1: four lanes per tile, the
 // reference's reduce tree per right-hand side; th > 1: two columns per lane,
 // fma chains over the brick-rows.  Normal blocks stage unit sums in LDS and
-// give each row to one lane; long rows keep per-lane sums.
+// give each row to one lane; long rows keep per-lane sums (FP32: with a
+// Neumaier compensation term, as the embedded-x kernels' long rows).
 // LDS of one wave for the column-map block code, sized for KC right-hand sides.
 template <typename AT, int KC>
 struct GatherLds {
@@ -151,9 +152,7 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
     const int lane = lane_id();
     const uint32_t nu = d.uend - d.ubeg, q = lane & 3, C = 2 * q;
     const __amdgpu_buffer_rsrc_t xr = make_xbuf(x, nrows, ldx); // unused unless XB
-    AT lacc[KC];
-#pragma unroll
-    for (int j = 0; j < KC; j++) lacc[j] = 0;
+    LaneSum<AT> lacc[KC]; // long rows; FP32 sums compensated (Neumaier)
     // tiles of this block; UG steps of 16 tiles are loaded before their
     // gathers are issued, so a lane keeps 2*UG gathers in flight
     constexpr int UG = KC >= 8 ? 1 : (KC >= 4 ? 2 : (KC == 1 ? UG1 : 4));
@@ -215,7 +214,7 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                 if (d.k != 0) {
                     if (ok)
 #pragma unroll
-                        for (int j = 0; j < KC; j++) lacc[j] = lacc[j] + (a0[u][j] + a1[u][j]);
+                        for (int j = 0; j < KC; j++) lacc[j].add(a0[u][j] + a1[u][j]);
                     continue;
                 }
                 const uint32_t key = kk[u][0];
@@ -241,8 +240,8 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                 if (d.k != 0) {
 #pragma unroll
                     for (int j = 0; j < KC; j++) {
-                        if (ok0) lacc[j] = lacc[j] + a0[u][j];
-                        if (ok1) lacc[j] = lacc[j] + a1[u][j];
+                        if (ok0) lacc[j].add(a0[u][j]);
+                        if (ok1) lacc[j].add(a1[u][j]);
                     }
                     continue;
                 }
@@ -262,7 +261,7 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
     if (d.k != 0) {
 #pragma unroll
         for (int j = 0; j < KC; j++) {
-            AT t = wave_sum<AT>(lacc[j]);
+            AT t = wave_sum<AT>(lacc[j].value());
             if (lane == 0 && (uint32_t)j < kv) {
                 if (d.k == 1) {
                     if (d.rbeg < nrows) y[(size_t)d.rbeg * ldy + j0 + j] = t;

@@ -72,6 +72,36 @@ def lib():
     return _lib
 
 
+_lib512 = None
+
+
+def host_has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return any(line.startswith("flags") and " avx512f" in line for line in f)
+    except OSError:
+        return False
+
+
+def lib_avx512():
+    """The AVX-512 CPU baseline (cpu_avx512.c: orc_bench_avx512, the
+    orc_bench protocol), or None when the host CPU has no AVX-512."""
+    global _lib512
+    if _lib512 is None:
+        if not host_has_avx512():
+            return None
+        path = os.path.join(HERE, "liboracle_avx512.so")
+        if not os.path.exists(path):
+            from . import build as _b
+            _b.build_avx512()
+        L = C.CDLL(path)
+        L.orc_bench_avx512.argtypes = [C.c_int, C.c_int, C.c_int, _u32, _u32, _u32, _vp, _vp,
+                                       _vp, _vp, _vp, _vp, _vp]
+        L.orc_bench_avx512.restype = C.c_double
+        _lib512 = L
+    return _lib512
+
+
 def _p(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
 

@@ -268,3 +268,56 @@ def test_multi_rhs_checker_equals_single_column():
     assert (i, j) == (123, 3) and e == per[3][0] == max(p[0] for p in per) and per[3][1] == 123
     Yp[0, 0] = np.nan
     assert O.scaled_err_mm(m.row, m.col, m.val, X, Y, Yp)[0] == np.inf
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4, 5, 6])
+def test_avx512_cpu_baseline_matches_oracle(kind):
+    """The AVX-512 CPU baseline gives the scalar oracle's results: bit for bit
+    where neither has atomics (cf_CSR, AXC, K1 and K1p: the same fused
+    chains and halving trees), to 1e-12 where y is accumulated atomically in
+    thread order (AXT), and COM with the reference's defects kept."""
+    import ctypes as C
+    L = O.lib_avx512()
+    if L is None:
+        pytest.skip("host CPU has no AVX-512")
+    m, x = _randn_matrix(3001, 9, 70)
+    y = np.zeros(m.nrows)
+    keep = []
+
+    def p(a):
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        return C.c_void_p(a.ctypes.data)
+    yr = O.cf_csr(m.row, m.col, m.val, x)
+    if kind == 0:
+        t = L.orc_bench_avx512(0, 1, 2, m.nrows, 0, 0, p(m.row), p(m.col), p(m.val), p(x), None,
+                               None, p(y))
+        ref, exact = yr, True
+    elif kind == 1:
+        a = H.AXC(m, x)
+        t = L.orc_bench_avx512(1, 3, 2, m.nrows, 0, 0, p(a.ax), p(a.brp), None, None, None, None,
+                               p(y))
+        ref, exact = O.int_axc(m.nrows, a.ax, a.brp), True
+    elif kind in (2, 3):
+        k = H.K1(m)
+        t = L.orc_bench_avx512(kind, 3, 2, m.nrows, k.chunkNum, 0, p(k.val), p(k.col), p(k.chp),
+                               p(k.nmc), p(x), p(k.permi), p(y))
+        ref, exact = yr, True  # K1 chains equal cf_CSR (test_lanewise_chains_are_fused)
+    elif kind in (4, 5):
+        th = 1 if kind == 4 else 4
+        a = H.AXT(m, x, th=th)
+        t = L.orc_bench_avx512(kind, 3, 2, m.nrows, a.tileN, th, p(a.ax), p(a.sec), None, None,
+                               None, None, p(y))
+        ref, exact = O.int_axt(a), False
+    else:
+        a = H.AXT(m, x, th=1, mode="COM", bs=128)
+        bn = (a.tileN * a.tileHW + 127) // 128
+        prm = np.array([128, a.log, a.tileHW, a.lenAX, a.lenSEC], np.uint32)
+        t = L.orc_bench_avx512(6, 3, 2, m.nrows, bn, 1, p(a.ax), p(a.sec), p(prm), None, None,
+                               None, p(y))
+        ref, exact = O.int_axt_com_h1(a), False
+    assert t > 0
+    if exact:
+        assert np.array_equal(y, ref), kind
+    else:
+        assert O.scaled_err(m.row, m.col, m.val, x, ref, y)[0] <= 1e-12, kind
