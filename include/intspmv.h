@@ -38,7 +38,8 @@ enum {
     INTSPMV_ERR_NOMEM = -3,       /* device or host allocation failed */
     INTSPMV_ERR_UNSUPPORTED = -4, /* combination not implemented */
     INTSPMV_ERR_OVERFLOW = -5,    /* a size does not fit the 32-bit index type */
-    INTSPMV_ERR_NODEVICE = -6     /* no HIP device visible */
+    INTSPMV_ERR_NODEVICE = -6,    /* no HIP device visible */
+    INTSPMV_ERR_TOO_LARGE = -7    /* beyond spmv_axt_h1_noalloc's fixed scratch */
 };
 
 /* element types of stored values / vectors */
@@ -221,6 +222,26 @@ int intspmv_batch_graph_launch(intspmv_ctx *ctx, const intspmv_batch *batch);
 int intspmv_batch_info(const intspmv_batch *batch, uint32_t *nitems, uint32_t *grid,
                        uint32_t *nsplit);
 int intspmv_batch_destroy(intspmv_batch *batch);
+
+/* ---- allocation-free product --------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc):
+ * the reference-layout product of spmv_axt for NOC, tile height 1, FP64
+ * values and x interleaved (int_axt_noc_h1, intSpmv_main.c:1385-1404), with
+ * no plan: y = A x from the builder's ax[tileN*16] and sec[tileN] alone.  It
+ * calls no hipMalloc / hipFree: rows that cross its fixed chunks of 64 tiles
+ * leave partial sums in a fixed device scratch array of
+ * INTSPMV_NOALLOC_SCRATCH_BYTES per device, set up by intspmv_create, which
+ * limits tileN to INTSPMV_NOALLOC_MAX_TILES (INTSPMV_ERR_TOO_LARGE beyond,
+ * nothing launched).  Every scratch entry a call reads is written earlier in
+ * the same call; calls that overlap in time on one GPU (several streams or
+ * contexts) share the array and race.  Rows inside one chunk get the bits of
+ * the serial tile order; rows across chunks add the chunks' partial sums in
+ * chunk order.  sec must be non-decreasing (as built). */
+#define INTSPMV_NOALLOC_SCRATCH_BYTES (8u << 20)
+#define INTSPMV_NOALLOC_MAX_TILES (INTSPMV_NOALLOC_SCRATCH_BYTES / 16u * 64u)
+int spmv_axt_h1_noalloc(intspmv_ctx *ctx, uint32_t nrows, uint32_t tileN, const double *ax,
+                        const uint32_t *sec, double *y);
 
 int intspmv_plan_destroy(intspmv_plan *plan);
 /* Host-only inspection of the work decomposition a plan uses for a row ->

@@ -8,9 +8,41 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
+
+// ---- scratch of the allocation-free entry point ---------------------------
+// spmv_axt_h1_noalloc keeps the partial sums of rows that cross its fixed tile
+// chunks here (two doubles per chunk of 64 tiles), so that it never allocates.
+// A file-scope device array: every device the library runs on holds its own
+// copy, at its own address, until the process ends.  In host code the name
+// `scratch` is not a device address: the address on each device comes from
+// hipGetSymbolAddress, taken once per device (with that device current) when
+// the first context on it is created, and only that pointer is used.
+static __device__ __align__(256) double scratch[INTSPMV_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[MAX_DEVICES];
+
+// The calling thread's current device must be `device`.
+static int scratch_address(int device, double **out, hipError_t *err)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= MAX_DEVICES) return INTSPMV_ERR_UNSUPPORTED;
+    if (!g_scratch_ptr[device]) {
+        double *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            *err = e;
+            return INTSPMV_ERR_HIP;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return INTSPMV_OK;
+}
 
 extern "C" const char *intspmv_strerror(int st)
 {
@@ -22,6 +54,9 @@ extern "C" const char *intspmv_strerror(int st)
     case INTSPMV_ERR_UNSUPPORTED: return "unsupported combination";
     case INTSPMV_ERR_OVERFLOW: return "size exceeds the 32-bit index range";
     case INTSPMV_ERR_NODEVICE: return "no HIP device";
+    case INTSPMV_ERR_TOO_LARGE:
+        return "matrix exceeds the allocation-free entry point's scratch "
+               "(INTSPMV_NOALLOC_MAX_TILES tiles)";
     default: return "unknown status";
     }
 }
@@ -74,6 +109,12 @@ extern "C" int intspmv_create(intspmv_ctx **ctx, int device)
     if (hipSetDevice(device) != hipSuccess) {
         free(c);
         return INTSPMV_ERR_HIP;
+    }
+    hipError_t e = hipSuccess;
+    int st = scratch_address(device, &c->noalloc_scratch, &e);
+    if (st) {
+        free(c);
+        return st;
     }
     *ctx = c;
     return INTSPMV_OK;

@@ -152,6 +152,7 @@ struct intspmv_ctx {
     int device;
     hipStream_t stream;
     hipError_t last_err;
+    double *noalloc_scratch; // this device's scratch array (spmv_axt_h1_noalloc)
 };
 
 enum PlanKind { PLAN_CSR = 1, PLAN_AXC = 2, PLAN_AXT = 3, PLAN_K1 = 4 };

@@ -86,6 +86,7 @@ SIGNATURES = [
     ("intspmv_batch_graph_launch", C.c_int, [_vp, _vp]),
     ("intspmv_batch_info", C.c_int, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
     ("intspmv_batch_destroy", C.c_int, [_vp]),
+    ("spmv_axt_h1_noalloc", C.c_int, [_vp, _u32, _u32, _vp, _vp, _vp]),
     ("intspmv_plan_destroy", C.c_int, [_vp]),
     ("intspmv_plan_info", C.c_int, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
     ("intspmv_plan_layout", C.c_int, [_u32, _vp, _u32, _u32, _u32, _u32, _vp, C.c_size_t,
