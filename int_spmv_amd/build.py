@@ -5,6 +5,7 @@ Produces, in-tree (so the libraries travel with the source tree):
   int_spmv_amd/lib/libintspmv.so       gfx950 HIP kernels behind the C ABI
   int_spmv_amd/lib/intspmv_main        C driver (the reference's main sequence)
   tests/c/adjoint_from_c               C caller of both libraries run by a GPU test
+  int_spmv_amd/lib_sanitized/          the C host library under ASan + UBSan (CPU tests)
 
 Run ``python -m int_spmv_amd.build`` or call :func:`build_all`.
 """
@@ -52,6 +53,28 @@ def build_host() -> str:
     return out
 
 
+# The C host library again, instrumented with AddressSanitizer and
+# UndefinedBehaviorSanitizer (host code only, built by gcc), for the CPU test
+# suite (tests/test_sanitizers.py); the GPU library beside it is the normal one.
+SANITIZED_LIBDIR = os.path.join(PKG, "lib_sanitized")
+SANITIZE_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fno-sanitize-recover=all",
+                  "-fsanitize=address,undefined"]
+
+
+def build_host_sanitized() -> str:
+    os.makedirs(SANITIZED_LIBDIR, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "host", "*.c")))
+    srcs = [s for s in srcs if not s.endswith("intspmv_main.c")]
+    out = os.path.join(SANITIZED_LIBDIR, "libintspmv_host.so")
+    flags = [f for f in HOST_FLAGS if f != "-O2"] + SANITIZE_FLAGS
+    _run([CC, *flags, "-shared", f"-I{INCLUDE}", *srcs, "-o", out + ".tmp", "-lm"])
+    os.replace(out + ".tmp", out)
+    dev = os.path.join(SANITIZED_LIBDIR, "libintspmv.so")
+    if not os.path.lexists(dev):
+        os.symlink(os.path.join("..", "lib", "libintspmv.so"), dev)
+    return out
+
+
 def build_device(outdir: str = LIBDIR, defines: tuple = ()) -> str:
     """defines: extra -D flags overriding kernel tunables (ISPV_* macros)."""
     os.makedirs(outdir, exist_ok=True)
@@ -93,10 +116,11 @@ def build_all() -> None:
     build_device()
     build_driver()
     build_c_tests()
+    build_host_sanitized()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"host": build_host, "device": build_device, "driver": build_driver,
-         "ctests": build_c_tests, "all": build_all}[w]()
+         "ctests": build_c_tests, "sanitized": build_host_sanitized, "all": build_all}[w]()
