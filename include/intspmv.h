@@ -160,9 +160,16 @@ int spmv_axt(intspmv_ctx *ctx, const intspmv_plan *plan, const void *ax, const v
 int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval,
                     const uint32_t *acol, const uint32_t *sec, const void *x, uint32_t ldx,
                     void *y, uint32_t ldy, uint32_t k, uint32_t kc);
-/* kc: right-hand sides per pass over the matrix (1, 2, 4 or 8, and 16 for F32
- * and BF16 values); 0 = for each pass the largest of these <= what is left
- * of k (16 only for F32/BF16). */
+/* kc: right-hand sides per pass over the matrix.  1, 2, 4, 8: narrow passes
+ * (lanes along the slots, each gathering its slot's kc values of x); 16:
+ * narrow for F32/BF16 values at any tile height; with tile height 1,
+ * kc >= 2 * (16 / sizeof x) -- 8, 16, 32 for F32 x; 4, 8, 16, 32 for F64 x --
+ * is a wide pass: lanes along the right-hand sides, a group of lanes reading
+ * each x row of kc values together, the matrix streamed once per pass (rows
+ * longer than a block take narrow passes of 8).  Both give the same bits for
+ * rows inside a block.  0 = automatic: at tile height 1 one wide pass of the
+ * smallest width >= what is left of k (up to 32), else the largest narrow
+ * width <= it. */
 /* The same product with the slot values read through a permutation: value
  * of slot s = vals[vperm[s]] (0 where vperm[s] = 0xFFFFFFFF; vperm 8-byte
  * aligned).  On the transposed tile map, with its slot map (A^T slot -> A
@@ -195,8 +202,10 @@ int spmv_axt_refresh_x(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_
 
 /* ---- batches ------------------------------------------------------------
  * Many AXT NOC matrices in column-map form, each with its own tile height,
- * 1..32 right-hand sides and right-hand-side blocking kc (1, 2, 4, 8; 0 = the
- * largest power of two <= min(k, 8)), run by ONE persistent kernel (plus one
+ * 1..32 right-hand sides and right-hand-side blocking kc (1, 2, 4, 8: narrow
+ * passes; at tile height 1 also the wide passes of spmv_axt_gather, kc >=
+ * 2 * (16 / sizeof x) up to 32; 0 = as spmv_axt_gather chooses), run by ONE
+ * persistent kernel (plus one
  * fix-up kernel when some row is longer than a wave streams alone).  The work
  * list is built once on the host.  intspmv_batch_graph_capture records the
  * launch in a hipGraph once; intspmv_batch_graph_launch replays it.  Pointers

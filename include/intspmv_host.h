@@ -172,12 +172,17 @@ int intspmv_axt_lengths(const intspmv_csr *m, uint32_t thw, uint32_t th, const c
 
 /* ---- autotuner model ------------------------------------------------------
  * Ranks AXT NOC tile heights th in {1, 2, 4, 8, 16} and right-hand-side
- * blockings kc in {1, 2, 4, 8} (kc <= k) for a column-map product with k
- * right-hand sides, sv-byte values and sx-byte x, by modelled bytes moved:
- *   passes * (slots * (sv + 4 + kc * sx) + 4 * lenSEC) + k * n * sx,
+ * blockings kc in {1, 2, 4, 8} (any th) and {16, 32} (th = 1, wide passes;
+ * FP64 x also takes 4 and 8 as wide passes at th = 1) for a column-map product
+ * with k right-hand sides, sv-byte values and sx-byte x (4 or 8), by modelled
+ * bytes moved:
+ *   passes * (slots * (sv + 4) + nnz * xline(th, kc) + 4 * lenSEC) + k * n * sx,
  * passes = ceil(k / kc), slots = lenAX / 2 from the exact length formula of
- * getArraysLenAXT (no arrays are built).  Fills up to cap entries of out in
- * ascending cost; *n receives the number of candidates. */
+ * getArraysLenAXT (no arrays are built), xline = the 64-byte lines one
+ * entry's x gathers touch per pass: one per 16-byte request of a lane (narrow)
+ * or the kc * sx bytes the lanes of a wide group read together.  Fills up to
+ * cap entries of out in ascending cost; *n receives the number of
+ * candidates (at most 30). */
 typedef struct {
     uint32_t tileH, kc;
     uint64_t slots, bytes;

@@ -27,7 +27,8 @@ struct BatchMat {
     const uint32_t *acol, *sec;
     const void *x;
     void *y;
-    uint32_t nrows, th, k, ldx, ldy, kc, piece_off, pad;
+    uint32_t nrows, th, k, ldx, ldy, kc, piece_off;
+    uint32_t xv; // x rows 16-byte aligned (wide passes load them whole)
 };
 struct BatchItem {
     uint32_t mat, block, j0, pad;
@@ -81,13 +82,46 @@ __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
         const BatchMat md = mats[item.mat];
         const BlockDesc d = blocks[item.block];
         double *part = partial + (size_t)md.piece_off * PARTIAL_STRIDE;
-        const uint32_t kv = md.k - item.j0 < md.kc ? md.k - item.j0 : md.kc;
         const VT *av = (const VT *)md.aval;
         const XT *xx = (const XT *)md.x;
         AT *yy = (AT *)md.y;
         GatherLds<AT, 8> &L = lds[w];
+        // wide passes for the normal blocks of tile height 1 (see
+        // gather_block_wide), narrow passes of at most 8 otherwise
+        constexpr uint32_t V = 16 / sizeof(XT);
+        // a wide pass keeps its round in the wave's narrow-pass value area
+        static_assert(sizeof(WideLds<AT, 32, XT>) <= sizeof(L.val) &&
+                          sizeof(WideLds<AT, 8, XT>) <= sizeof(L.val),
+                      "a wide round fits the wave's LDS");
+        if (md.th == 1 && d.k == 0 && md.kc >= wide_min_kc<XT>()) {
+            const uint32_t kv = md.k - item.j0 < md.kc ? md.k - item.j0 : md.kc;
+#define ISPV_BW(KW)                                                                             \
+    if (md.kc == KW) {                                                                          \
+        if (md.xv)                                                                              \
+            gather_block_wide<VT, XT, AT, KW, true>(d, av, md.acol, md.sec, xx, md.ldx, yy,     \
+                                                    md.ldy, item.j0, kv, md.nrows, L.key,       \
+                                                    L.seg,                                      \
+                                                    *reinterpret_cast<WideLds<AT, KW, XT> *>(L.val)); \
+        else                                                                                    \
+            gather_block_wide<VT, XT, AT, KW, false>(d, av, md.acol, md.sec, xx, md.ldx, yy,    \
+                                                     md.ldy, item.j0, kv, md.nrows, L.key,      \
+                                                     L.seg,                                     \
+                                                     *reinterpret_cast<WideLds<AT, KW, XT> *>(L.val)); \
+    }
+            if constexpr (V == 2) {
+                ISPV_BW(4)
+            }
+            ISPV_BW(8)
+            ISPV_BW(16)
+            ISPV_BW(32)
+#undef ISPV_BW
+            wave_lds_sync(); // the next item reuses this wave's LDS
+            continue;
+        }
+        const uint32_t kcn = md.kc > 8 ? 8 : md.kc;
+        const uint32_t kv = md.k - item.j0 < kcn ? md.k - item.j0 : kcn;
 #define ISPV_B(KC)                                                                              \
-    if (md.kc == KC) {                                                                          \
+    if (kcn == KC) {                                                                            \
         if (md.th == 1)                                                                         \
             gather_block<VT, XT, AT, KC, true, XB, ISPV_B_UG1>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy,  \
                                                item.j0, kv, 1, md.nrows, part, L.val, L.key, L.seg); \
@@ -152,8 +186,20 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
         if (th == 0 || M.k == 0 || M.k > PARTIAL_STRIDE || M.ldx < M.k || M.ldy < M.k ||
             (M.nrows && (!M.con_host || !M.x || !M.y)))
             return INTSPMV_ERR_ARG;
-        uint32_t kc = M.kc ? M.kc : (M.k >= 8 ? 8 : (M.k >= 4 ? 4 : (M.k >= 2 ? 2 : 1)));
-        if (kc != 1 && kc != 2 && kc != 4 && kc != 8) return INTSPMV_ERR_ARG;
+        // kc 1, 2, 4, 8: narrow passes; at tile height 1, kc >= 2 * (16 / sizeof
+        // x) up to 32: wide passes (long rows in narrow passes of 8); 0: as
+        // spmv_axt_gather chooses
+        const uint32_t vx = x_type == INTSPMV_F64 ? 2 : 4;
+        const uint32_t wmin = x_type == INTSPMV_F64 ? wide_min_kc<double>() : wide_min_kc<float>();
+        uint32_t kc = M.kc;
+        if (!kc && th == 1 && M.k >= wmin && wmin <= 32) {
+            kc = wmin;
+            while (kc < M.k && kc < 32) kc *= 2;
+        }
+        if (!kc) kc = M.k >= 8 ? 8 : (M.k >= 4 ? 4 : (M.k >= 2 ? 2 : 1));
+        const bool wide = th == 1 && kc >= wmin;
+        if (kc != 1 && kc != 2 && kc != 4 && kc != 8 && !(wide && (kc == 16 || kc == 32)))
+            return INTSPMV_ERR_ARG;
         const uint32_t umax = M.umax ? M.umax : ISPV_AXT_UMAX;
         if (umax > ISPV_AXT_UMAX || umax < 8) return INTSPMV_ERR_ARG;
         const uint32_t ths = 8 * th;
@@ -174,15 +220,18 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
         int st = ispv_build_blocks(M.nrows, uptr.data(), umax, BLOCK_ROWS_MAX, long_units,
                                    th == 1 ? 1024 : 8192 / th, mb, ms, nlw, nlong, npieces);
         if (st) return st;
-        hm[i] = {M.aval, M.acol, M.sec, M.x, M.y, M.nrows, th, M.k, M.ldx, M.ldy, kc, piece_base, 0};
+        const uint32_t xv = M.ldx % vx == 0 && (uintptr_t)M.x % 16 == 0;
+        hm[i] = {M.aval, M.acol, M.sec, M.x, M.y, M.nrows, th, M.k, M.ldx, M.ldy, kc, piece_base, xv};
         xb = xb && xbuf_fits(M.nrows, M.ldx, x_type == INTSPMV_F64 ? 8 : 4);
         for (const BlockDesc &d : mb) {
             const uint32_t bi = (uint32_t)blocks.size();
             blocks.push_back(d);
             if (d.rbeg == 0xFFFFFFFFu) continue; // XCD padding of the per-matrix plan
             const uint64_t slots = (uint64_t)(d.uend - d.ubeg) * (th == 1 ? 8 : th);
-            for (uint32_t j0 = 0; j0 < M.k; j0 += kc)
-                cand.push_back({{i, bi, j0, 0}, slots * (kc < M.k - j0 ? kc : M.k - j0) + 64});
+            // long rows: narrow passes of min(kc, 8), as the kernel's kcn
+            const uint32_t step = (wide && d.k != 0 && kc > 8) ? 8 : kc;
+            for (uint32_t j0 = 0; j0 < M.k; j0 += step)
+                cand.push_back({{i, bi, j0, 0}, slots * (step < M.k - j0 ? step : M.k - j0) + 64});
         }
         for (const SplitDesc &sp : ms) splits.push_back({i, sp.row, sp.b0, sp.k});
         piece_base += npieces;

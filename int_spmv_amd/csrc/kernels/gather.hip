@@ -42,6 +42,27 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
                                                        vperm);
 }
 
+// Wide passes (tile height 1): the normal blocks of the plan only (the
+// launch passes the normal section of the block list); long rows keep the
+// narrow passes.  See gather_block_wide.
+template <typename VT, typename XT, typename AT, int KW, bool XV>
+__global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_wide_kernel(
+    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ aval,
+    const uint32_t *__restrict__ acol, const uint32_t *__restrict__ sec,
+    const XT *__restrict__ x, uint32_t ldx, AT *__restrict__ y, uint32_t ldy, uint32_t j0,
+    uint32_t kv, uint32_t nrows)
+{
+    __shared__ uint32_t s_key[WAVES][G_UMAX];
+    __shared__ uint16_t s_seg[WAVES][G_UMAX + 1];
+    __shared__ WideLds<AT, KW, XT> s_wide[WAVES];
+    const uint32_t bid = block_index(0);
+    if (bid >= nblocks) return;
+    const BlockDesc d = blocks[bid];
+    const uint32_t w = threadIdx.x / WAVE;
+    gather_block_wide<VT, XT, AT, KW, XV>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv, nrows,
+                                          s_key[w], s_seg[w], s_wide[w]);
+}
+
 // A_bar[s] = y_bar[row(s)] * x[acol[s]] (0 on padding).  row(s) is the sec
 // word of the slot's tile (th == 1) or column (th > 1).  Two slots per thread.
 template <typename VT, typename XT, typename AT>
@@ -133,29 +154,34 @@ __global__ __launch_bounds__(256) void refresh_x_kernel(uint32_t nslots, uint32_
     *reinterpret_cast<d2v *>(ax + (T * th + w / 8) * 16 + 8 + w % 8) = o;
 }
 
-template <typename VT, typename XT, typename AT, bool PV = false>
-static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t *acol,
-                         const uint32_t *sec, const void *x, uint32_t ldx, void *y, uint32_t ldy,
-                         uint32_t k, uint32_t kcf, const uint32_t *vperm = nullptr)
+// Right-hand sides per wide pass for the rest of k (0: take narrow passes).
+template <typename XT>
+__host__ __device__ constexpr uint32_t wide_width(uint32_t rem)
 {
-    const uint32_t nb = p->nblocks, th = p->axt.tileH;
-    if (nb == 0) return INTSPMV_OK;
-    dim3 grid(ispv_grid(p)), blk(BLOCK_THREADS);
+    if (rem < wide_min_kc<XT>() || wide_min_kc<XT>() > 32) return 0;
+    uint32_t kw = wide_min_kc<XT>();
+    while (kw < rem && kw < 32) kw *= 2;
+    return kw;
+}
+
+// Narrow passes of kc right-hand sides from j0 over blocks [b0, b0 + nb) of
+// the plan's list (the long section is the first nlong_wg * WAVES blocks).
+template <typename VT, typename XT, typename AT, bool PV>
+static void launch_narrow(const intspmv_plan *p, const BlockDesc *blocks, uint32_t nlong_wg,
+                          uint32_t nb, dim3 grid, const void *aval, const uint32_t *acol,
+                          const uint32_t *sec, const void *x, uint32_t ldx, void *y,
+                          uint32_t ldy, uint32_t j0, uint32_t kc, uint32_t kv,
+                          const uint32_t *vperm)
+{
+    const uint32_t th = p->axt.tileH;
     hipStream_t s = p->ctx->stream;
-    const uint32_t nl = p->nlong_wg;
+    dim3 blk(BLOCK_THREADS);
     // buffer-resource gathers for passes of fewer right-hand sides than one
     // 16-byte load holds (with more, the plain-load kernel measured faster)
-    const bool xb_ok = xbuf_fits(p->nrows, ldx, sizeof(XT));
-    for (uint32_t j0 = 0; j0 < k;) {
-        const uint32_t rem = k - j0;
-        const uint32_t kc = kcf ? kcf
-                                : (rem >= 16 && sizeof(AT) == 4) ? 16
-                                : (rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1)));
-        const uint32_t kv = rem < kc ? rem : kc;
-        const bool xb = xb_ok && kc < 16 / sizeof(XT);
+    const bool xb = xbuf_fits(p->nrows, ldx, sizeof(XT)) && kc < 16 / sizeof(XT);
 #define ISPV_GX(KC, H1, XB)                                                                     \
     axt_gather_kernel<VT, XT, AT, KC, H1, XB, PV><<<grid, blk, 0, s>>>(                         \
-        p->d_blocks, nl, nb, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y, ldy, j0, \
+        blocks, nlong_wg, nb, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y, ldy, j0, \
         kv, th, p->nrows, p->d_partial, vperm)
 #define ISPV_G(KC)                                                                              \
     if (kc == KC) {                                                                             \
@@ -172,17 +198,80 @@ static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t
             else ISPV_GX(KC, false, false);                                                     \
         }                                                                                       \
     }
-        ISPV_G(1)
-        ISPV_G(2)
-        ISPV_G(4)
-        ISPV_G(8)
-        if constexpr (sizeof(AT) == 4) {
-            ISPV_G(16)
-        }
+    ISPV_G(1)
+    ISPV_G(2)
+    ISPV_G(4)
+    ISPV_G(8)
+    if constexpr (sizeof(AT) == 4) {
+        ISPV_G(16)
+    }
 #undef ISPV_G
 #undef ISPV_GX
-        int st = ispv_launch_check(p->ctx);
-        if (st) return st;
+}
+
+template <typename VT, typename XT, typename AT, bool PV = false>
+static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t *acol,
+                         const uint32_t *sec, const void *x, uint32_t ldx, void *y, uint32_t ldy,
+                         uint32_t k, uint32_t kcf, const uint32_t *vperm = nullptr)
+{
+    const uint32_t nb = p->nblocks, th = p->axt.tileH;
+    if (nb == 0) return INTSPMV_OK;
+    hipStream_t s = p->ctx->stream;
+    const uint32_t nl = p->nlong_wg, nlb = nl * WAVES; // long section: blocks [0, nlb)
+    constexpr uint32_t V = 16 / sizeof(XT);
+    const bool wide_ok = th == 1 && !PV;
+    // 16-byte loads of x rows: rows and pass starts aligned to V values
+    const bool xv = ldx % V == 0 && (uintptr_t)x % 16 == 0;
+    for (uint32_t j0 = 0; j0 < k;) {
+        const uint32_t rem = k - j0;
+        uint32_t kc = kcf;
+        if (!kc) {
+            kc = wide_ok ? wide_width<XT>(rem) : 0;
+            if (!kc)
+                kc = (rem >= 16 && sizeof(AT) == 4) ? 16
+                   : (rem >= 8 ? 8 : (rem >= 4 ? 4 : (rem >= 2 ? 2 : 1)));
+        }
+        const uint32_t kv = rem < kc ? rem : kc;
+        if (wide_ok && kc >= wide_min_kc<XT>()) {
+            // normal blocks: one wide pass; long rows: narrow passes of min(kc, 8)
+            if (nb > nlb) {
+                const uint32_t nn = nb - nlb;
+                dim3 grid((nn + WAVES - 1) / WAVES), blk(BLOCK_THREADS);
+#define ISPV_GW(KW)                                                                             \
+    if (kc == KW) {                                                                             \
+        if (xv)                                                                                 \
+            axt_gather_wide_kernel<VT, XT, AT, KW, true><<<grid, blk, 0, s>>>(                  \
+                p->d_blocks + nlb, nn, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y, \
+                ldy, j0, kv, p->nrows);                                                         \
+        else                                                                                    \
+            axt_gather_wide_kernel<VT, XT, AT, KW, false><<<grid, blk, 0, s>>>(                 \
+                p->d_blocks + nlb, nn, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y, \
+                ldy, j0, kv, p->nrows);                                                         \
+    }
+                if constexpr (V == 2) {
+                    ISPV_GW(4)
+                }
+                ISPV_GW(8)
+                ISPV_GW(16)
+                ISPV_GW(32)
+#undef ISPV_GW
+                int st = ispv_launch_check(p->ctx);
+                if (st) return st;
+            }
+            const uint32_t kl = kc > 8 ? 8 : kc; // as the batch kernel's long rows
+            for (uint32_t jj = 0; nl && jj < kv; jj += kl) {
+                launch_narrow<VT, XT, AT, PV>(p, p->d_blocks, nl, nlb, dim3(nl), aval, acol, sec,
+                                              x, ldx, y, ldy, j0 + jj, kl,
+                                              kv - jj < kl ? kv - jj : kl, vperm);
+                int st = ispv_launch_check(p->ctx);
+                if (st) return st;
+            }
+        } else {
+            launch_narrow<VT, XT, AT, PV>(p, p->d_blocks, nl, nb, dim3(ispv_grid(p)), aval, acol,
+                                          sec, x, ldx, y, ldy, j0, kc, kv, vperm);
+            int st = ispv_launch_check(p->ctx);
+            if (st) return st;
+        }
         j0 += kc;
     }
     return ispv_split_fixup(p, y, sizeof(AT) == 4, k, ldy, 0);
@@ -198,6 +287,19 @@ static bool gather_plan_ok(const intspmv_ctx *ctx, const intspmv_plan *plan)
            plan->axt.mode == INTSPMV_NOC;
 }
 
+// kc: 0 (automatic), 1, 2, 4, 8 (narrow passes), 16 (narrow for FP32 / BF16
+// values, or wide), 32 (wide); wide passes need tile height 1 and values read
+// directly (not through a permutation), and at least two lanes per x row.
+static bool gather_kc_ok(const intspmv_plan *plan, uint32_t kc, bool permuted)
+{
+    if (kc == 0 || kc == 1 || kc == 2 || kc == 4 || kc == 8) return true;
+    const bool f64x = plan->axt.x_type == INTSPMV_F64;
+    const uint32_t wmin = f64x ? wide_min_kc<double>() : wide_min_kc<float>();
+    const bool wide = plan->axt.tileH == 1 && !permuted && kc >= wmin;
+    if (kc == 16) return wide || !f64x;
+    return kc == 32 && wide;
+}
+
 extern "C" int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval,
                                const uint32_t *acol, const uint32_t *sec, const void *x,
                                uint32_t ldx, void *y, uint32_t ldy, uint32_t k, uint32_t kc)
@@ -205,9 +307,7 @@ extern "C" int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const
     // the column-map blocks stage at most G_UMAX units: a plan built for the
     // reference-layout kernels with larger blocks is refused
     if (!gather_plan_ok(ctx, plan) || plan->umax > G_UMAX || k == 0 || k > PARTIAL_STRIDE ||
-        ldx < k || ldy < k ||
-        (kc != 0 && kc != 1 && kc != 2 && kc != 4 && kc != 8 &&
-         !(kc == 16 && plan->axt.val_type != INTSPMV_F64)))
+        ldx < k || ldy < k || !gather_kc_ok(plan, kc, false))
         return INTSPMV_ERR_ARG;
     if (plan->nrows && (!x || !y || (plan->nunits && (!aval || !acol || !sec))))
         return INTSPMV_ERR_ARG;
@@ -230,9 +330,7 @@ extern "C" int spmv_axt_gather_permuted(intspmv_ctx *ctx, const intspmv_plan *pl
                                         uint32_t kc)
 {
     if (!gather_plan_ok(ctx, plan) || plan->umax > G_UMAX || k == 0 || k > PARTIAL_STRIDE ||
-        ldx < k || ldy < k ||
-        (kc != 0 && kc != 1 && kc != 2 && kc != 4 && kc != 8 &&
-         !(kc == 16 && plan->axt.val_type != INTSPMV_F64)))
+        ldx < k || ldy < k || !gather_kc_ok(plan, kc, true))
         return INTSPMV_ERR_ARG;
     if (plan->nrows && (!x || !y || (plan->nunits && (!vals || !vperm || !acol || !sec))))
         return INTSPMV_ERR_ARG;

@@ -311,4 +311,198 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
     });
 }
 
+// ---- wide passes: lanes along the right-hand sides ------------------------
+// For a pass of KW right-hand sides over a normal block of tile height 1.  A
+// group of G = KW / V lanes (V = values per 16-byte load of x: 4 FP32, 2 FP64)
+// takes one tile at a time; lane l of a group owns right-hand sides
+// j0 + V l .. j0 + V l + V - 1.  For each slot the group's lanes read the
+// slot's x row of KW values together (one cache line for 32 FP32 or 16 FP64
+// values) instead of every lane reading its own slot's row, and the block's
+// values and columns are read once for all KW right-hand sides.  Tiles are
+// dealt round-robin to the 64 / G groups, wide_u per group per round; a
+// round's tile sums (the reference's tree of the eight products, per
+// right-hand side) go to LDS, and one lane per right-hand side then adds them
+// to its running row sum in tile order, carrying the row across rounds --
+// tiles of a row added in order from +0.0, the bits of the narrow passes.
+// Tiles per group per round: two for FP64 x (V = 2), one for FP32 x, whose
+// four values per slot and lane fill the registers of a second tile's
+// gathers (measured: FP64 stencil k = 8 420 against 491 us with one; FP32
+// stencil k = 32 607 against 765 us with two).
+#ifdef ISPV_WIDE_U
+template <typename XT>
+constexpr int wide_u() { return ISPV_WIDE_U; }
+#else
+template <typename XT>
+constexpr int wide_u() { return sizeof(XT) == 8 ? 2 : 1; }
+#endif
+// Smallest group (lanes per x row) that takes wide passes: a pass of kc
+// right-hand sides at tile height 1 is wide when kc >= ISPV_WIDE_MIN_G * V.
+#ifndef ISPV_WIDE_MIN_G
+#define ISPV_WIDE_MIN_G 2
+#endif
+template <typename XT>
+__host__ __device__ constexpr uint32_t wide_min_kc()
+{
+    return ISPV_WIDE_MIN_G * (16 / sizeof(XT));
+}
+template <int KW, typename XT>
+struct WideShape {
+    static constexpr int V = 16 / (int)sizeof(XT);
+    static constexpr int U = wide_u<XT>();
+    static constexpr int G = KW / V, NG = WAVE / G, R = U * NG; // tiles per round
+    static_assert(G >= 2 && WAVE % G == 0 && R <= WAVE, "wide passes: 2+ lanes per row");
+};
+// LDS of one wave for a wide pass: a round's tile sums sv[R][KW], the carried
+// row sums carry[KW] and the round's run starts rs[R + 1].
+template <typename AT, int KW, typename XT>
+struct WideLds {
+    AT sv[WideShape<KW, XT>::R * KW];
+    AT carry[KW];
+    uint16_t rs[WideShape<KW, XT>::R + 1];
+};
+
+// The eight slot values of tile T as AT.
+template <typename VT, typename AT>
+__device__ __forceinline__ void tile_values(const VT *aval, size_t T, AT (&v)[8])
+{
+    if constexpr (sizeof(VT) == 8) {
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            const d2v t = ntload(reinterpret_cast<const d2v *>(aval + T * 8 + q));
+            v[q] = (AT)t[0];
+            v[q + 1] = (AT)t[1];
+        }
+    } else if constexpr (sizeof(VT) == 4) {
+        typedef float f4v __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int q = 0; q < 8; q += 4) {
+            const f4v t = ntload(reinterpret_cast<const f4v *>(aval + T * 8 + q));
+#pragma unroll
+            for (int e = 0; e < 4; e++) v[q + e] = (AT)t[e];
+        }
+    } else {
+        typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+        const u4v t = ntload(reinterpret_cast<const u4v *>(aval + T * 8));
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            v[2 * e] = (AT)__uint_as_float(t[e] << 16);
+            v[2 * e + 1] = (AT)__uint_as_float(t[e] & 0xFFFF0000u);
+        }
+    }
+}
+
+// V values of x row c from right-hand side j (0 for padding and for the
+// values at or past cnt = right-hand sides left for this lane), without
+// branches, so that the eight gathers of a tile stay in flight together.
+// XV (rows and j aligned to V, ldx a multiple of V): one 16-byte load, in
+// bounds since j < k <= ldx makes j + V <= ldx; a lane with nothing left
+// (cnt <= 0) reads the row's first V values instead, and padding row 0.
+// Otherwise one load per value, indexes clamped the same way.
+template <typename XT, typename AT, int V, bool XV>
+__device__ __forceinline__ void wide_x(const XT *x, uint32_t c, uint32_t ldx, uint32_t j, int cnt,
+                                       AT (&o)[V])
+{
+    const bool ok = c != PAD;
+    const XT *p = x + (size_t)(ok ? c : 0) * ldx + (cnt > 0 ? j : 0);
+    if constexpr (XV) {
+        if constexpr (sizeof(XT) == 8) {
+            const d2v t = *reinterpret_cast<const d2v *>(p);
+#pragma unroll
+            for (int e = 0; e < V; e++) o[e] = (ok && e < cnt) ? (AT)t[e] : (AT)0;
+        } else {
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v t = *reinterpret_cast<const f4v *>(p);
+#pragma unroll
+            for (int e = 0; e < V; e++) o[e] = (ok && e < cnt) ? (AT)t[e] : (AT)0;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < V; e++) {
+            const XT t = p[e < cnt ? e : 0];
+            o[e] = (ok && e < cnt) ? (AT)t : (AT)0;
+        }
+    }
+}
+
+template <typename VT, typename XT, typename AT, int KW, bool XV>
+__device__ __forceinline__ void gather_block_wide(const BlockDesc &d, const VT *__restrict__ aval,
+                                                  const uint32_t *__restrict__ acol,
+                                                  const uint32_t *__restrict__ sec,
+                                                  const XT *__restrict__ x, uint32_t ldx,
+                                                  AT *__restrict__ y, uint32_t ldy, uint32_t j0,
+                                                  uint32_t kv, uint32_t nrows, uint32_t *sk,
+                                                  uint16_t *sg, WideLds<AT, KW, XT> &W)
+{
+    using S = WideShape<KW, XT>;
+    constexpr int V = S::V, G = S::G, NG = S::NG, R = S::R, U = S::U;
+    const int lane = lane_id();
+    const uint32_t grp = lane / G, jl = (lane % G) * V;
+    const int cnt = (int)kv - (int)jl; // right-hand sides left for this lane (<= 0: none)
+    const uint32_t nu = d.uend - d.ubeg;
+    for (uint32_t i = lane; i < nu; i += WAVE) sk[i] = ntload(sec + d.ubeg + i);
+    wave_lds_sync();
+    const uint32_t nseg = find_segments(sk, sg, nu);
+    for (uint32_t b0 = 0; b0 < nu; b0 += R) {
+        // tile sums of this round: group grp takes tiles b0 + grp + NG u
+        AT ts[U][V];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t t = b0 + grp + NG * u;
+            const size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+            AT v[8];
+            tile_values<VT, AT>(aval, T, v);
+            typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+            const u4v c0 = ntload(reinterpret_cast<const u4v *>(acol + T * 8));
+            const u4v c1 = ntload(reinterpret_cast<const u4v *>(acol + T * 8 + 4));
+            AT xv[8][V];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                wide_x<XT, AT, V, XV>(x, c0[q], ldx, j0 + jl, cnt, xv[q]);
+                wide_x<XT, AT, V, XV>(x, c1[q], ldx, j0 + jl, cnt, xv[q + 4]);
+            }
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                AT pr[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) pr[q] = v[q] * xv[q][e];
+                ts[u][e] = ((pr[0] + pr[4]) + (pr[2] + pr[6])) + ((pr[1] + pr[5]) + (pr[3] + pr[7]));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (b0 + grp + NG * u < nu)
+#pragma unroll
+                for (int e = 0; e < V; e++) W.sv[(grp + NG * u) * KW + jl + e] = ts[u][e];
+        // runs of equal rows among the round's tiles (a run at the round's
+        // start may continue the previous round's row, the last may go on)
+        const uint32_t here = nu - b0 < (uint32_t)R ? nu - b0 : (uint32_t)R;
+        const uint32_t t = b0 + lane;
+        const bool start = (uint32_t)lane < here && (lane == 0 || sk[t] != sk[t - 1]);
+        const uint64_t m = __ballot(start);
+        if (start) W.rs[__popcll(m & lanemask_lt())] = (uint16_t)lane;
+        const uint32_t nr = __popcll(m);
+        if (lane == 0) W.rs[nr] = (uint16_t)here;
+        const bool first_cont = b0 > 0 && sk[b0] == sk[b0 - 1];
+        const bool last_cont = b0 + here < nu && sk[b0 + here] == sk[b0 + here - 1];
+        wave_lds_sync();
+        // one lane per (run, right-hand side): the run's tile sums in order,
+        // from the carried sum when the run continues a row, from +0.0 else
+        for (uint32_t p = lane; p < nr * KW; p += WAVE) {
+            const uint32_t ri = p / KW, j = p % KW, a = W.rs[ri], b = W.rs[ri + 1];
+            AT acc = (ri == 0 && first_cont) ? W.carry[j] : (AT)0;
+            for (uint32_t i = a; i < b; i++) acc = acc + W.sv[i * KW + j];
+            if (ri + 1 == nr && last_cont) {
+                W.carry[j] = acc;
+            } else {
+                const uint32_t r = sk[b0 + a];
+                if (r < nrows && j < kv) y[(size_t)r * ldy + j0 + j] = acc;
+            }
+        }
+        wave_lds_sync(); // the next round overwrites sv and rs
+    }
+    zero_empty_rows(sk, sg, nseg, d, nrows, [&](uint32_t z) {
+        for (uint32_t j = 0; j < kv; j++) y[(size_t)z * ldy + j0 + j] = (AT)0;
+    });
+}
+
 } // namespace ispv

@@ -437,9 +437,9 @@ class AxtTransposeMap:
 
 def tune_model(m: CSR, k: int, sv: int = 8, sx: int = 8) -> list[dict]:
     """Candidate (tileH, kc) layouts ranked by modelled bytes (intspmv_tune_axt)."""
-    out = (TuneChoice * 20)()
+    out = (TuneChoice * 32)()
     n = C.c_uint32()
-    _check(lib().intspmv_tune_axt(C.byref(m._s), k, sv, sx, out, 20, C.byref(n)), "tune")
+    _check(lib().intspmv_tune_axt(C.byref(m._s), k, sv, sx, out, 32, C.byref(n)), "tune")
     return [dict(tileH=o.tileH, kc=o.kc, slots=o.slots, bytes=o.bytes) for o in out[: n.value]]
 
 

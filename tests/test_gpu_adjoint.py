@@ -104,9 +104,9 @@ def test_gather_wide_x_plain_loads(engine, mats, vt):
 
 @pytest.mark.parametrize("vt", [D.F32, D.BF16])
 def test_gather_sixteen_per_pass(engine, mats, vt):
-    """FP32 sums take 16 right-hand sides per pass (the default for k >= 16):
-    each right-hand side keeps its own sums in the same order, so the bits
-    equal passes of 8; FP64 refuses 16."""
+    """Passes of 16 right-hand sides (narrow at tile height 4, wide at tile
+    height 1) against passes of 8: each right-hand side keeps its own sums in
+    the same order, so the bits are equal; FP64 refuses narrow passes of 16."""
     import torch
     for name, m in mats[:2]:
         n = m.nrows
@@ -125,14 +125,18 @@ def test_gather_sixteen_per_pass(engine, mats, vt):
                 yr = O.cf_csr(m.row, m.col, m.val, X[:, j].astype(np.float64))
                 tol = 1e-6 if vt == D.F32 else 1e-2
                 assert scaled(m, X[:, j].astype(np.float64), yr, Yg[:, j]) <= tol, (name, th, k, j)
+    # FP64 has no narrow pass of 16 (tile height 4 refuses it); at tile height
+    # 1 kc = 16 is a wide pass
     m = mats[0][1]
-    a = H.AXT(m, np.zeros(m.nrows), th=1, want_acol=True)
-    g = D.DeviceAXTGather(engine, a)
-    Y = _nan(engine, m.nrows * 16)
-    st = D.lib().spmv_axt_gather(engine.h, g.plan.h, D._dptr(g.aval), D._dptr(g.acol),
-                                 D._dptr(g.sec), D._dptr(engine.upload(np.zeros(m.nrows * 16))),
-                                 16, D._dptr(Y), 16, 16, 16)
-    assert st == -1
+    for th, want in ((4, -1), (1, 0)):
+        a = H.AXT(m, np.zeros(m.nrows), th=th, want_acol=True)
+        g = D.DeviceAXTGather(engine, a)
+        Y = _nan(engine, m.nrows * 16)
+        st = D.lib().spmv_axt_gather(engine.h, g.plan.h, D._dptr(g.aval), D._dptr(g.acol),
+                                     D._dptr(g.sec),
+                                     D._dptr(engine.upload(np.zeros(m.nrows * 16))), 16,
+                                     D._dptr(Y), 16, 16, 16)
+        assert st == want, th
 
 
 @pytest.mark.parametrize("vt,xt,tol", [(D.F32, D.F32, 1e-6), (D.BF16, D.F32, 1e-2)])
@@ -319,3 +323,41 @@ def test_adjoint_from_c(th):
                        timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "paths_differ 0" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("vt", [D.F64, D.F32, D.BF16])
+def test_wide_passes_unaligned_rows(engine, mats, vt):
+    """Wide passes (tile height 1) load each x row with 16-byte loads when
+    rows are aligned, one value at a time otherwise (here an odd leading
+    dimension), with lanes past k idle: the same bits either way, equal to
+    narrow passes of 2 for rows inside a block, and within tolerance of
+    cf_CSR everywhere."""
+    import torch
+    m = mats[0][1]
+    n = m.nrows
+    xt = D.F64 if vt == D.F64 else D.F32
+    dt = torch.float64 if xt == D.F64 else torch.float32
+    a = H.AXT(m, np.zeros(n), th=1, want_acol=True)
+    g = D.DeviceAXTGather(engine, a, vt, xt)
+    for k in (7, 13, 20):
+        X = np.stack([H.uniform(n, seed=90 + j) for j in range(k)], axis=1)
+        X = X.astype(np.float32).astype(np.float64) if xt == D.F32 else X
+        Xa = torch.tensor(X.reshape(-1), dtype=dt, device=engine.torch_device)
+        Xo = torch.zeros((n, k + 1), dtype=dt, device=engine.torch_device)  # odd rows
+        Xo[:, :k] = Xa.view(n, k)
+        Ya, Yo, Y2 = (torch.full((n * k,), float("nan"), dtype=dt, device=engine.torch_device)
+                      for _ in range(3))
+        g.matvec(Xa, Ya, k=k)
+        g.matvec(Xo.view(-1), Yo, k=k, ldx=k + 1)
+        g.matvec(Xa, Y2, k=k, kc=2)
+        torch.cuda.synchronize()
+        ya, yo, y2 = (t.double().cpu().numpy().reshape(n, k) for t in (Ya, Yo, Y2))
+        assert bitwise_equal(ya, yo), (vt, k)
+        units = (np.diff(m.row.astype(np.int64)) + 7) // 8
+        ok = units <= 64  # rows in the plan's normal blocks
+        assert bitwise_equal(ya[ok], y2[ok]), (vt, k)
+        vals = m.val if vt == D.F64 else (m.val.astype(np.float32).astype(np.float64) if vt == D.F32
+                                          else D.bf16_to_f32(D.bf16_bits(m.val)).astype(np.float64))
+        Yr = O.csr_mm(m.row, m.col, vals, X)
+        tol = 1e-12 if vt == D.F64 else 1e-6
+        assert O.scaled_err_mm(m.row, m.col, vals, X, Yr, ya)[0] <= tol, (vt, k)

@@ -119,16 +119,24 @@ def test_bad_arguments_are_refused_before_launch(engine):
     X = torch.rand(n * 40, dtype=torch.float64, device=engine.torch_device)
     Y = _nan(engine, n * 40)
     bad = [dict(k=0), dict(k=33), dict(k=4, ldx=3), dict(k=4, ldy=3), dict(k=4, kc=3),
-           dict(k=16, kc=16), dict(k=8, kc=32)]
+           dict(k=8, kc=64)]
     for kw in bad:
         with pytest.raises(D.IntSpmvError) as ei:
             g64.matvec(X, Y, **kw)
+        assert ei.value.status == -1, kw
+    # FP64 passes of 16 or 32 are wide passes, which need tile height 1
+    a4 = H.AXT(m, np.zeros(n), th=4, want_acol=True)
+    g4 = D.DeviceAXTGather(engine, a4)
+    for kw in (dict(k=16, kc=16), dict(k=8, kc=32)):
+        with pytest.raises(D.IntSpmvError) as ei:
+            g4.matvec(X, Y, **kw)
         assert ei.value.status == -1, kw
     with pytest.raises(D.IntSpmvError) as ei:
         g64.matvec(None, Y)
     assert ei.value.status == -1
     ent = dict(a=a, X=X[:n * 4], Y=Y[:n * 4], k=4)
-    for kw in (dict(kc=16), dict(kc=3), dict(ldx=3), dict(umax=4), dict(umax=256)):
+    for kw in (dict(kc=64), dict(kc=3), dict(ldx=3), dict(umax=4), dict(umax=256),
+               dict(a=a4, kc=16), dict(a=a4, kc=32)):
         with pytest.raises(D.IntSpmvError) as ei:
             D.DeviceBatch(engine, [dict(ent, **kw)], D.F64, D.F64)
         assert ei.value.status == -1, kw
