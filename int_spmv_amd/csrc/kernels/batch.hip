@@ -44,25 +44,43 @@ struct intspmv_batch {
     int device; // the context's device, so destroying needs nothing of the context
     int val_type, x_type;
     uint32_t nmat, nitems, nblocks, nsplit, grid;
+    uint32_t nnarrow, grid_w; // items [0, nnarrow) narrow passes, the rest wide; wide grid
     bool xb; // every x spans less than 2 GiB: buffer-resource gathers
     ispv::BatchMat *d_mats;
     ispv::BatchItem *d_items;
     ispv::BlockDesc *d_blocks;
     ispv::BatchSplit *d_splits;
     double *d_partial;
-    uint32_t *d_xcd; // item ranges of the XCDs: [xcd[i], xcd[i+1])
+    uint32_t *d_xcd; // item ranges of the XCDs: narrow [xcd[i], xcd[i+1]), wide from xcd + 9
     hipGraph_t graph;
     hipGraphExec_t exec;
     hipStream_t cap_stream;
+    hipStream_t aux;             // the wide kernel's stream (default priority)
+    hipEvent_t fork, join;       // orders aux after / before the caller's stream
 };
 
 
 namespace ispv {
 
+// Groups of 16 tiles in flight per step in narrow passes of one and of four
+// right-hand sides.  The narrow and the wide passes run as two kernels (PATHS
+// 1 and 2) so that each keeps its own registers: in one kernel more groups in
+// flight sped the k = 1 and k = 4 matrices of the 256-matrix batch
+// (396 -> 316 us, 587 -> 487 us) but slowed the k = 32 ones (852 -> 990 us).
 #ifndef ISPV_B_UG1
-#define ISPV_B_UG1 1 // one right-hand side: one group of 16 tiles per step (fewer registers)
+#define ISPV_B_UG1 8
 #endif
-template <typename VT, typename XT, typename AT, bool XB>
+#ifndef ISPV_B_UG4
+#define ISPV_B_UG4 4
+#endif
+// FP64 sums keep 4 and 2 (twice the registers per value: 8 and 4 would take
+// the narrow kernel to 220 VGPRs, two waves per SIMD)
+template <typename AT>
+constexpr int batch_ug1() { return sizeof(AT) == 8 ? 4 : ISPV_B_UG1; }
+template <typename AT>
+constexpr int batch_ug4() { return sizeof(AT) == 8 ? 2 : ISPV_B_UG4; }
+enum { BATCH_NARROW = 1, BATCH_WIDE = 2 };
+template <typename VT, typename XT, typename AT, bool XB, int PATHS>
 __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
     const BatchItem *__restrict__ items, const uint32_t *__restrict__ xcd_range,
     const BlockDesc *__restrict__ blocks, const BatchMat *__restrict__ mats, double *partial)
@@ -86,14 +104,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
         const XT *xx = (const XT *)md.x;
         AT *yy = (AT *)md.y;
         GatherLds<AT, 8> &L = lds[w];
-        // wide passes for the normal blocks of tile height 1 (see
-        // gather_block_wide), narrow passes of at most 8 otherwise
-        constexpr uint32_t V = 16 / sizeof(XT);
-        // a wide pass keeps its round in the wave's narrow-pass value area
+        // PATHS = BATCH_WIDE: wide passes (normal blocks of tile height 1, see
+        // gather_block_wide), keeping a round in the wave's narrow-pass value
+        // area; BATCH_NARROW: narrow passes of at most 8 right-hand sides
         static_assert(sizeof(WideLds<AT, 32, XT>) <= sizeof(L.val) &&
                           sizeof(WideLds<AT, 8, XT>) <= sizeof(L.val),
                       "a wide round fits the wave's LDS");
-        if (md.th == 1 && d.k == 0 && md.kc >= wide_min_kc<XT>()) {
+        if constexpr (PATHS & BATCH_WIDE) {
             const uint32_t kv = md.k - item.j0 < md.kc ? md.k - item.j0 : md.kc;
 #define ISPV_BW(KW)                                                                             \
     if (md.kc == KW) {                                                                          \
@@ -108,25 +125,24 @@ __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
                                                      L.seg,                                     \
                                                      *reinterpret_cast<WideLds<AT, KW, XT> *>(L.val)); \
     }
-            if constexpr (V == 2) {
+            if constexpr (sizeof(XT) == 8) {
                 ISPV_BW(4)
             }
             ISPV_BW(8)
             ISPV_BW(16)
             ISPV_BW(32)
 #undef ISPV_BW
-            wave_lds_sync(); // the next item reuses this wave's LDS
-            continue;
         }
+        if constexpr (PATHS & BATCH_NARROW) {
         const uint32_t kcn = md.kc > 8 ? 8 : md.kc;
         const uint32_t kv = md.k - item.j0 < kcn ? md.k - item.j0 : kcn;
 #define ISPV_B(KC)                                                                              \
     if (kcn == KC) {                                                                            \
         if (md.th == 1)                                                                         \
-            gather_block<VT, XT, AT, KC, true, XB, ISPV_B_UG1>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy,  \
+            gather_block<VT, XT, AT, KC, true, XB, batch_ug1<AT>(), false, batch_ug4<AT>()>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy,  \
                                                item.j0, kv, 1, md.nrows, part, L.val, L.key, L.seg); \
         else                                                                                    \
-            gather_block<VT, XT, AT, KC, false, XB, ISPV_B_UG1>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy, \
+            gather_block<VT, XT, AT, KC, false, XB, batch_ug1<AT>(), false, batch_ug4<AT>()>(d, av, md.acol, md.sec, xx, md.ldx, yy, md.ldy, \
                                                 item.j0, kv, md.th, md.nrows, part, L.val, L.key,  \
                                                 L.seg);                                         \
     }
@@ -135,6 +151,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
         ISPV_B(4)
         ISPV_B(8)
 #undef ISPV_B
+        }
         wave_lds_sync(); // the next item reuses this wave's LDS
     }
 }
@@ -230,29 +247,53 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
             const uint64_t slots = (uint64_t)(d.uend - d.ubeg) * (th == 1 ? 8 : th);
             // long rows: narrow passes of min(kc, 8), as the kernel's kcn
             const uint32_t step = (wide && d.k != 0 && kc > 8) ? 8 : kc;
-            for (uint32_t j0 = 0; j0 < M.k; j0 += step)
-                cand.push_back({{i, bi, j0, 0}, slots * (step < M.k - j0 ? step : M.k - j0) + 64});
+            for (uint32_t j0 = 0; j0 < M.k; j0 += step) {
+                // modelled bytes of the item (as the autotuner's model): the
+                // stream plus the cache lines of its x gathers
+                const uint32_t kv = step < M.k - j0 ? step : M.k - j0;
+                const uint64_t xs = x_type == INTSPMV_F64 ? 8 : 4;
+                const bool wi = wide && d.k == 0;
+                const uint64_t xline = wi ? (kv * xs + 63) / 64 * 64 : (kv * xs + 15) / 16 * 64;
+                cand.push_back({{i, bi, j0, 0}, slots * (8 + xline) + 256});
+            }
         }
         for (const SplitDesc &sp : ms) splits.push_back({i, sp.row, sp.b0, sp.k});
         piece_base += npieces;
     }
-    // items stay in (matrix, block, first RHS) order; the XCDs get contiguous
-    // ranges of equal cost
-    std::vector<BatchItem> items(cand.size());
-    uint64_t cost_all = 0;
-    for (size_t i = 0; i < cand.size(); i++) {
-        items[i] = cand[i].item;
-        cost_all += cand[i].cost;
-    }
-    std::vector<uint32_t> xcd_range(NUM_XCD + 1, (uint32_t)cand.size());
-    xcd_range[0] = 0;
-    {
+    // Items stay in (matrix, block, first RHS) order: the narrow passes first,
+    // then the wide ones (normal blocks of tile height 1 with a wide kc), each
+    // list cut into contiguous per-XCD ranges of equal cost.
+    std::vector<BatchItem> items;
+    std::vector<uint32_t> xcd_range(2 * (NUM_XCD + 1));
+    uint32_t nnarrow = 0;
+    uint64_t cost_n = 0, cost_w = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        std::vector<const Cand *> sel;
+        uint64_t cost_all = 0;
+        for (const Cand &c : cand) {
+            const BatchMat &md = hm[c.item.mat];
+            const uint32_t wmin =
+                x_type == INTSPMV_F64 ? wide_min_kc<double>() : wide_min_kc<float>();
+            const bool w = md.th == 1 && blocks[c.item.block].k == 0 && md.kc >= wmin;
+            if (w == (pass == 1)) {
+                sel.push_back(&c);
+                cost_all += c.cost;
+            }
+        }
+        const uint32_t base = (uint32_t)items.size();
+        uint32_t *xr = &xcd_range[pass * (NUM_XCD + 1)];
+        for (int x = 0; x <= NUM_XCD; x++) xr[x] = base + (uint32_t)sel.size();
+        xr[0] = base;
         uint64_t acc = 0;
         int x = 1;
-        for (size_t i = 0; i < cand.size() && x < NUM_XCD; i++) {
-            acc += cand[i].cost;
-            while (x < NUM_XCD && acc * NUM_XCD >= cost_all * (uint64_t)x) xcd_range[x++] = (uint32_t)(i + 1);
+        for (size_t i = 0; i < sel.size(); i++) {
+            items.push_back(sel[i]->item);
+            acc += sel[i]->cost;
+            while (x < NUM_XCD && acc * NUM_XCD >= cost_all * (uint64_t)x)
+                xr[x++] = base + (uint32_t)(i + 1);
         }
+        if (pass == 0) nnarrow = (uint32_t)items.size();
+        (pass == 0 ? cost_n : cost_w) = cost_all;
     }
 
     intspmv_batch *b = (intspmv_batch *)calloc(1, sizeof(intspmv_batch));
@@ -263,6 +304,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
     b->x_type = x_type;
     b->nmat = nmat;
     b->nitems = (uint32_t)items.size();
+    b->nnarrow = nnarrow;
     b->nblocks = (uint32_t)blocks.size();
     b->nsplit = (uint32_t)splits.size();
     int st = ispv_enter(ctx);
@@ -270,18 +312,25 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
         free(b);
         return st;
     }
-    int dev = ctx->device, cus = 0, per_cu = 0;
+    int dev = ctx->device, cus = 0, per_n = 0, per_w = 0;
     hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     b->xb = xb;
 #define ISPV_OCC(VT, XT, AT)                                                                    \
-    (xb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<VT, XT, AT, true>,  \
-                                                       BLOCK_THREADS, 0)                         \
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<VT, XT, AT, false>, \
-                                                       BLOCK_THREADS, 0))
+    do {                                                                                        \
+        if (xb)                                                                                 \
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                   \
+                &per_n, batch_kernel<VT, XT, AT, true, BATCH_NARROW>, BLOCK_THREADS, 0);        \
+        else                                                                                    \
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                   \
+                &per_n, batch_kernel<VT, XT, AT, false, BATCH_NARROW>, BLOCK_THREADS, 0);       \
+        if (e == hipSuccess)                                                                    \
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                   \
+                &per_w, batch_kernel<VT, XT, AT, false, BATCH_WIDE>, BLOCK_THREADS, 0);         \
+    } while (0)
     if (e == hipSuccess) {
-        if (f64) e = ISPV_OCC(double, double, double);
-        else if (f32) e = ISPV_OCC(float, float, float);
-        else e = ISPV_OCC(uint16_t, float, float);
+        if (f64) ISPV_OCC(double, double, double);
+        else if (f32) ISPV_OCC(float, float, float);
+        else ISPV_OCC(uint16_t, float, float);
     }
 #undef ISPV_OCC
     if (e != hipSuccess || cus <= 0) {
@@ -289,10 +338,23 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
         free(b);
         return INTSPMV_ERR_HIP;
     }
-    if (per_cu < 1) per_cu = 1;
-    uint32_t want = (b->nitems + WAVES - 1) / WAVES;
-    b->grid = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)(cus * per_cu), want));
-    b->grid = (b->grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD; // whole rounds over the XCDs
+    // grids: the resident capacity (whole rounds over the XCDs), no more
+    // waves than items
+    auto grid_for = [&](int per_cu, uint32_t n) {
+        per_cu = per_cu < 1 ? 1 : per_cu;
+        uint32_t want = (n + WAVES - 1) / WAVES;
+        uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)(cus * per_cu), want));
+        return (g + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    };
+    // The two kernels run at the same time (fork / join on a second stream),
+    // each on a share of the device in proportion to its modelled bytes, so
+    // both finish together instead of one after the other.
+    const double fn = (cost_n + cost_w) ? (double)cost_n / (double)(cost_n + cost_w) : 0.0;
+    b->grid = grid_for((int)std::max(1.0, per_n * fn + 0.5), nnarrow);
+    b->grid_w = grid_for((int)std::max(1.0, per_w * (1.0 - fn) + 0.5), b->nitems - nnarrow);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
     auto up = [&](void **dst, const void *src, size_t bytes) {
         if (e != hipSuccess) return;
         e = hipMalloc(dst, bytes ? bytes : 16);
@@ -317,14 +379,25 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
 
 static int batch_enqueue(const intspmv_batch *b, hipStream_t s)
 {
+    const bool both = b->nnarrow && b->nitems > b->nnarrow;
+    hipStream_t sw = both ? b->aux : s; // the wide kernel's stream
+    if (both && (hipEventRecord(b->fork, s) != hipSuccess ||
+                 hipStreamWaitEvent(b->aux, b->fork, 0) != hipSuccess))
+        return INTSPMV_ERR_HIP;
     if (b->nitems) {
+        const uint32_t *xw = b->d_xcd + NUM_XCD + 1;
 #define ISPV_BL(VT, XT, AT)                                                                     \
-    if (b->xb)                                                                                  \
-        batch_kernel<VT, XT, AT, true><<<b->grid, BLOCK_THREADS, 0, s>>>(                        \
-            b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);                         \
-    else                                                                                        \
-        batch_kernel<VT, XT, AT, false><<<b->grid, BLOCK_THREADS, 0, s>>>(                       \
-            b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial)
+    if (b->nnarrow) {                                                                           \
+        if (b->xb)                                                                              \
+            batch_kernel<VT, XT, AT, true, BATCH_NARROW><<<b->grid, BLOCK_THREADS, 0, s>>>(      \
+                b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);                     \
+        else                                                                                    \
+            batch_kernel<VT, XT, AT, false, BATCH_NARROW><<<b->grid, BLOCK_THREADS, 0, s>>>(     \
+                b->d_items, b->d_xcd, b->d_blocks, b->d_mats, b->d_partial);                     \
+    }                                                                                           \
+    if (b->nitems > b->nnarrow)                                                                 \
+        batch_kernel<VT, XT, AT, false, BATCH_WIDE><<<b->grid_w, BLOCK_THREADS, 0, sw>>>(        \
+            b->d_items, xw, b->d_blocks, b->d_mats, b->d_partial)
         if (b->val_type == INTSPMV_F64) {
             ISPV_BL(double, double, double);
         } else if (b->val_type == INTSPMV_F32) {
@@ -334,6 +407,9 @@ static int batch_enqueue(const intspmv_batch *b, hipStream_t s)
         }
 #undef ISPV_BL
     }
+    if (both && (hipEventRecord(b->join, b->aux) != hipSuccess ||
+                 hipStreamWaitEvent(s, b->join, 0) != hipSuccess))
+        return INTSPMV_ERR_HIP;
     if (b->nsplit) {
         dim3 g((b->nsplit * PARTIAL_STRIDE + 255) / 256);
         if (b->val_type == INTSPMV_F64)
@@ -405,6 +481,9 @@ extern "C" int intspmv_batch_destroy(intspmv_batch *b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    if (b->aux) (void)hipStreamDestroy(b->aux);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
     (void)hipFree(b->d_mats);
     (void)hipFree(b->d_items);
     (void)hipFree(b->d_blocks);

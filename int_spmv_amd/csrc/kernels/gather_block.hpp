@@ -133,13 +133,12 @@ struct GatherLds {
     uint16_t seg[G_UMAX + 1];
 };
 
-// UG1: groups of 16 tiles in flight per step for one right-hand side (the
-// single-matrix kernel keeps more gathers in flight; the batched kernel,
-// whose registers are those of its widest path, fewer).
+// UG1 / UG4: groups of 16 tiles in flight per step for passes of one / four
+// right-hand sides (the kernels choose them by measurement).
 // PV: slot values read through a permutation, value of slot s =
 // aval[vperm[s]] (0 where vperm[s] is PAD) -- Aᵀ's map reading A's values.
 template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB, int UG1 = 4,
-          bool PV = false>
+          bool PV = false, int UG4 = 2>
 __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__restrict__ aval,
                                              const uint32_t *__restrict__ acol,
                                              const uint32_t *__restrict__ sec,
@@ -155,7 +154,7 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
     LaneSum<AT> lacc[KC]; // long rows; FP32 sums compensated (Neumaier)
     // tiles of this block; UG steps of 16 tiles are loaded before their
     // gathers are issued, so a lane keeps 2*UG gathers in flight
-    constexpr int UG = KC >= 8 ? 1 : (KC >= 4 ? 2 : (KC == 1 ? UG1 : 4));
+    constexpr int UG = KC >= 8 ? 1 : (KC >= 4 ? UG4 : (KC == 1 ? UG1 : 4));
     const uint32_t T0 = H1 ? d.ubeg : d.ubeg / 8, T1 = H1 ? d.uend : (d.uend + 7) / 8;
     for (uint32_t Tbb = T0; Tbb < T1; Tbb += 16 * UG) {
         AT a0[UG][KC], a1[UG][KC];
