@@ -575,6 +575,7 @@ def main():
         # the slot map, or first copied into Aᵀ's order (a permutation)
         xb.fill_(float("nan"))
         w, e = timed(lambda: adj.adjoint(yb, xb, vals=g.aval), 20, 3, torch)
+        perm_t = e
         record("AXT_ADJOINT_PERMUTED_H1_F64",
                M.bytes_axt_gather(n, at.lenAX, at.lenSEC) + (at.lenAX // 2) * 4, w, e,
                {"vs_forward_gather": round(e / fwd_g, 3),
@@ -588,10 +589,25 @@ def main():
         log(f"[bench] AXT_ADJOINT_VALUES_PATH_H1_F64: {formats['AXT_ADJOINT_VALUES_PATH_H1_F64']}")
         vb = nanv(ag.lenAX // 2)
         w, e = timed(lambda: g.grad_vals(xd, yb, vb), 20, 3, torch)
+        gv_t = e
         record("AXT_GRAD_VALS_H1_F64", (ag.lenAX // 2) * (4 + 8) + ag.lenSEC * 4 + 2 * n * 8, w, e,
                {"vs_forward_gather": round(e / fwd_g, 3)})
         keep("AXT_GRAD_VALS_H1_F64", "gv", vb, ag.lenAX // 2, ybar=ybh,
              nzslot=ag.nzslot.copy())
+        # values held in both slot orders (the autograd pair operator): x_bar
+        # on A^T's own copy with A_bar in A^T's order written in the same pass
+        # (spmv_axt_gather_grad); backward = that + A_bar in A's order
+        vbt = nanv(at.lenAX // 2)
+        xb.fill_(float("nan"))
+        w, e = timed(lambda: adj.adjoint_grad(yb, xb, xd, vbt), 20, 3, torch)
+        record("AXT_ADJOINT_GRAD_H1_F64",
+               M.bytes_axt_gather(n, at.lenAX, at.lenSEC) + (at.lenAX // 2) * 8 + n * 8, w, e,
+               {"vs_forward_gather": round(e / fwd_g, 3),
+                "vs_forward_embedded": round(e / head_ev, 3),
+                "backward_pair_us": round((e + gv_t) * 1e6, 2),
+                "backward_permuted_us": round((perm_t + gv_t) * 1e6, 2)})
+        keep("AXT_ADJOINT_GRAD_H1_F64", "adj", xb, n, ybar=ybh)
+        del vbt
         del g, adj, X8, Y8, vb
         # the same in FP32 storage (values and x FP32, FP32 sums)
         g32 = D.DeviceAXTGather(eng, ag, D.F32, D.F32)

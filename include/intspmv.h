@@ -184,6 +184,18 @@ int spmv_axt_gather_permuted(intspmv_ctx *ctx, const intspmv_plan *plan, const v
  * padding), one entry per slot (lenAX/2).  F64 or F32. */
 int spmv_axt_grad_vals(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_t *acol,
                        const uint32_t *sec, const void *x, const void *ybar, void *vbar);
+/* On the transposed tile map of A (intspmv_axt_transpose_map) with A^T's own
+ * copy of the values: x_bar = A^T y_bar (one right-hand side, the product of
+ * spmv_axt_gather) and, in the same pass, A's value gradient in A^T's slot
+ * order, vbar[s] = y_bar[acol[s]] * x[row(s)] (0 on padding) -- the value of
+ * spmv_axt_grad_vals for the entry slot s holds, bit for bit.  For callers
+ * that keep A's values in both slot orders (updated alike by an elementwise
+ * optimiser), so that the adjoint never reads A's values through the slot
+ * map.  F64 and F32; vbar 16-byte (F64) / 8-byte (F32) aligned.  No reference
+ * counterpart. */
+int spmv_axt_gather_grad(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval,
+                         const uint32_t *acol, const uint32_t *sec, const void *ybar, void *xbar,
+                         const void *x, void *vbar);
 /* Same for the reference layout, whose tiles already hold x[col]:
  * vbar[s] = ybar[row(s)] * (x half of slot s). */
 int spmv_axt_grad_vals_embedded(intspmv_ctx *ctx, const intspmv_plan *plan, const double *ax,

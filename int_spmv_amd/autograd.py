@@ -7,6 +7,14 @@ y = A(vals) x with A an AXT NOC matrix in column-map form.  Backward:
                            tune_adjoint finds that faster)
   vals_bar = y_bar[row(s)] * x[col(s)] per slot -- spmv_axt_grad_vals
 Both are hand-written gfx950 kernels; torch only holds the tensors.
+
+AxtSpMVPair holds A's values twice, in A's slot order (read by the forward)
+and in A^T's (read by x_bar = A^T y_bar), as two leaves: its backward returns
+the value gradient in both orders -- the A^T-order one written by the adjoint
+kernel in the same pass (spmv_axt_gather_grad) -- so an elementwise optimiser
+updates the two copies alike, entry for entry, and the adjoint never reads
+A's values through the slot map.  (A global-norm computation over the
+parameters counts the pair twice.)
 """
 from __future__ import annotations
 
@@ -36,6 +44,12 @@ class AxtOperator:
         """The matrix values in slot order (a leaf tensor to optimise)."""
         return self.fwd.aval.clone()
 
+    def initial_values_pair(self):
+        """The matrix values in A's and in A^T's slot order (two leaves for
+        AxtSpMVPair)."""
+        self.adj.refresh(self.fwd.aval)
+        return self.fwd.aval.clone(), self.adj.gT.aval.clone()
+
 
 class AxtSpMV(torch.autograd.Function):
     @staticmethod
@@ -64,5 +78,38 @@ class AxtSpMV(torch.autograd.Function):
         return gv, gx, None
 
 
+class AxtSpMVPair(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vals, vals_t, x, op: AxtOperator):
+        vals, vals_t, x = vals.contiguous(), vals_t.contiguous(), x.contiguous()
+        op.eng.follow_current_stream()
+        y = torch.empty(op.m.nrows, dtype=op.dtype, device=x.device)
+        op.fwd.matvec(x, y, vals=vals)
+        ctx.save_for_backward(vals, vals_t, x)
+        ctx.op = op
+        return y
+
+    @staticmethod
+    def backward(ctx, ybar):
+        vals, vals_t, x = ctx.saved_tensors
+        op = ctx.op
+        ybar = ybar.contiguous()
+        op.eng.follow_current_stream()
+        gv = gvt = gx = None
+        if ctx.needs_input_grad[0]:
+            gv = torch.empty_like(vals)
+            op.fwd.grad_vals(x, ybar, gv)
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            gx, gvt = torch.empty_like(x), torch.empty_like(vals_t)
+            op.adj.adjoint_grad(ybar, gx, x, gvt, vals_t=vals_t)
+            gx = gx if ctx.needs_input_grad[2] else None
+            gvt = gvt if ctx.needs_input_grad[1] else None
+        return gv, gvt, gx, None
+
+
 def spmv(vals, x, op: AxtOperator):
     return AxtSpMV.apply(vals, x, op)
+
+
+def spmv_pair(vals, vals_t, x, op: AxtOperator):
+    return AxtSpMVPair.apply(vals, vals_t, x, op)

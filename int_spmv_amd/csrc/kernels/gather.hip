@@ -24,22 +24,24 @@ namespace ispv {
 #define ISPV_G_UG1 8 // one right-hand side: 8 groups of 16 tiles, 16 gathers per lane in flight
 #endif
 
-template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB, bool PV>
+template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB, bool PV,
+          bool GR = false>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
     const VT *__restrict__ aval, const uint32_t *__restrict__ acol,
     const uint32_t *__restrict__ sec, const XT *__restrict__ x, uint32_t ldx,
     AT *__restrict__ y, uint32_t ldy, uint32_t j0, uint32_t kv, uint32_t th, uint32_t nrows,
-    double *partial, const uint32_t *__restrict__ vperm)
+    double *partial, const uint32_t *__restrict__ vperm, const XT *__restrict__ xg = nullptr,
+    VT *__restrict__ gvt = nullptr)
 {
     __shared__ GatherLds<AT, KC> lds[WAVES];
     const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
     GatherLds<AT, KC> &L = lds[threadIdx.x / WAVE];
-    gather_block<VT, XT, AT, KC, H1, XB, ISPV_G_UG1, PV>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv,
-                                                       th, nrows, partial, L.val, L.key, L.seg,
-                                                       vperm);
+    gather_block<VT, XT, AT, KC, H1, XB, ISPV_G_UG1, PV, 2, GR>(
+        d, aval, acol, sec, x, ldx, y, ldy, j0, kv, th, nrows, partial, L.val, L.key, L.seg,
+        vperm, xg, gvt);
 }
 
 // Wide passes (tile height 1): the normal blocks of the plan only (the
@@ -277,6 +279,53 @@ static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t
     return ispv_split_fixup(p, y, sizeof(AT) == 4, k, ldy, 0);
 }
 
+// Slots of the tile columns no block covers -- the unoccupied columns
+// [colN, tileN * 8) at the end of the last tile (th > 1) -- are padding: 0.
+template <typename VT>
+__global__ __launch_bounds__(64) void tail_columns_zero_kernel(VT *__restrict__ g, uint32_t tileN,
+                                                               uint32_t th, uint32_t colN)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, c = colN + i / th, R = i % th;
+    if (c >= tileN * 8) return;
+    g[((size_t)(c / 8) * th + R) * 8 + c % 8] = (VT)0;
+}
+
+// X_bar = A^T Y_bar (one right-hand side) with A's value gradient in this
+// map's slot order written in the same pass (GR).
+template <typename VT, typename XT, typename AT>
+static int launch_gather_grad(const intspmv_plan *p, const void *aval, const uint32_t *acol,
+                              const uint32_t *sec, const void *ybar, void *xbar, const void *x,
+                              void *gvt)
+{
+    const uint32_t nb = p->nblocks, th = p->axt.tileH;
+    if (nb == 0) return INTSPMV_OK;
+    dim3 grid(ispv_grid(p)), blk(BLOCK_THREADS);
+    hipStream_t s = p->ctx->stream;
+    const bool xb = xbuf_fits(p->nrows, 1, sizeof(XT));
+#define ISPV_GG(H1, XB)                                                                         \
+    axt_gather_kernel<VT, XT, AT, 1, H1, XB, false, true><<<grid, blk, 0, s>>>(                 \
+        p->d_blocks, p->nlong_wg, nb, (const VT *)aval, acol, sec, (const XT *)ybar, 1,          \
+        (AT *)xbar, 1, 0, 1, th, p->nrows, p->d_partial, nullptr, (const XT *)x, (VT *)gvt)
+    if (th == 1) {
+        if (xb) ISPV_GG(true, true);
+        else ISPV_GG(true, false);
+    } else {
+        if (xb) ISPV_GG(false, true);
+        else ISPV_GG(false, false);
+    }
+#undef ISPV_GG
+    int st = ispv_launch_check(p->ctx);
+    if (st) return st;
+    const uint32_t colN = p->axt.colN, cols = p->axt.tileN * 8;
+    if (th > 1 && colN < cols) {
+        tail_columns_zero_kernel<VT><<<((cols - colN) * th + 63) / 64, 64, 0, s>>>(
+            (VT *)gvt, p->axt.tileN, th, colN);
+        st = ispv_launch_check(p->ctx);
+        if (st) return st;
+    }
+    return ispv_split_fixup(p, xbar, sizeof(AT) == 4, 1, 1, 0);
+}
+
 } // namespace ispv
 
 using namespace ispv;
@@ -344,6 +393,25 @@ extern "C" int spmv_axt_gather_permuted(intspmv_ctx *ctx, const intspmv_plan *pl
     if (D.val_type == INTSPMV_F32 && D.x_type == INTSPMV_F32)
         return launch_gather<float, float, float, true>(plan, vals, acol, sec, x, ldx, y, ldy, k,
                                                         kc, vperm);
+    return INTSPMV_ERR_UNSUPPORTED;
+}
+
+extern "C" int spmv_axt_gather_grad(intspmv_ctx *ctx, const intspmv_plan *plan,
+                                    const void *aval, const uint32_t *acol, const uint32_t *sec,
+                                    const void *ybar, void *xbar, const void *x, void *vbar)
+{
+    if (!gather_plan_ok(ctx, plan) || plan->umax > G_UMAX) return INTSPMV_ERR_ARG;
+    if (plan->nrows && (!ybar || !xbar || !x || (plan->nunits && (!aval || !acol || !sec || !vbar))))
+        return INTSPMV_ERR_ARG;
+    if (plan->nunits && ((uintptr_t)vbar % (plan->axt.val_type == INTSPMV_F64 ? 16 : 8)))
+        return INTSPMV_ERR_ARG; // pairs of slots are stored together
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    const intspmv_axt_desc &D = plan->axt;
+    if (D.val_type == INTSPMV_F64 && D.x_type == INTSPMV_F64)
+        return launch_gather_grad<double, double, double>(plan, aval, acol, sec, ybar, xbar, x, vbar);
+    if (D.val_type == INTSPMV_F32 && D.x_type == INTSPMV_F32)
+        return launch_gather_grad<float, float, float>(plan, aval, acol, sec, ybar, xbar, x, vbar);
     return INTSPMV_ERR_UNSUPPORTED;
 }
 

@@ -137,8 +137,13 @@ struct GatherLds {
 // right-hand sides (the kernels choose them by measurement).
 // PV: slot values read through a permutation, value of slot s =
 // aval[vperm[s]] (0 where vperm[s] is PAD) -- Aᵀ's map reading A's values.
+// GR (one right-hand side, the transposed map of a matrix A): the same pass
+// also writes the value gradient of A in this map's slot order,
+// gvt[s] = x[acol[s]] * xg[row(s)] (0 on padding) -- with x = y_bar and
+// xg = the forward's x, A_bar of the entry slot s holds, the product of
+// spmv_axt_grad_vals in the same operand order, so the same bits.
 template <typename VT, typename XT, typename AT, int KC, bool H1, bool XB, int UG1 = 4,
-          bool PV = false, int UG4 = 2>
+          bool PV = false, int UG4 = 2, bool GR = false>
 __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__restrict__ aval,
                                              const uint32_t *__restrict__ acol,
                                              const uint32_t *__restrict__ sec,
@@ -146,8 +151,12 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                                              AT *__restrict__ y, uint32_t ldy, uint32_t j0,
                                              uint32_t kv, uint32_t th, uint32_t nrows,
                                              double *partial, AT *sv, uint32_t *sk,
-                                             uint16_t *sg, const uint32_t *__restrict__ vperm = nullptr)
+                                             uint16_t *sg, const uint32_t *__restrict__ vperm = nullptr,
+                                             const XT *__restrict__ xg = nullptr,
+                                             VT *__restrict__ gvt = nullptr)
 {
+    static_assert(!GR || (KC == 1 && !std::is_same<VT, uint16_t>::value),
+                  "the value gradient: one right-hand side, FP64 / FP32 values");
     const int lane = lane_id();
     const uint32_t nu = d.uend - d.ubeg, q = lane & 3, C = 2 * q;
     const __amdgpu_buffer_rsrc_t xr = make_xbuf(x, nrows, ldx); // unused unless XB
@@ -193,6 +202,36 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
                 AT x0[KC], x1[KC];
                 gather_x<XT, AT, KC, XB>(x, xr, cc[u][0], ldx, j0, kv, x0);
                 gather_x<XT, AT, KC, XB>(x, xr, cc[u][1], ldx, j0, kv, x1);
+                if constexpr (GR) {
+                    const uint32_t Tr = Tbb + 16 * u + lane / 4;
+                    const size_t T = Tr < T1 ? Tr : T1 - 1;
+                    const size_t sidx = (T * th + R) * 8 + C;
+                    // slots of this block only (th > 1 blocks may share a tile)
+                    const uint32_t u0 = 8 * Tr + C;
+                    const bool ok0 = Tr < T1 && (H1 || (u0 >= d.ubeg && u0 < d.uend));
+                    const bool ok1 = Tr < T1 && (H1 || (u0 + 1 >= d.ubeg && u0 + 1 < d.uend));
+                    const uint32_t r0 = kk[u][0], r1 = H1 ? kk[u][0] : kk[u][1];
+                    const AT xa = r0 < nrows ? (AT)xg[r0] : (AT)0;
+                    const AT xb = r1 < nrows ? (AT)xg[r1] : (AT)0;
+                    const AT g0 = cc[u][0] != PAD ? x0[0] * xa : (AT)0;
+                    const AT g1 = cc[u][1] != PAD ? x1[0] * xb : (AT)0;
+                    if (ok0 && ok1) {
+                        if constexpr (sizeof(VT) == 8) {
+                            d2v o;
+                            o[0] = g0;
+                            o[1] = g1;
+                            __builtin_nontemporal_store(o, reinterpret_cast<d2v *>(gvt + sidx));
+                        } else {
+                            f2v o;
+                            o[0] = g0;
+                            o[1] = g1;
+                            __builtin_nontemporal_store(o, reinterpret_cast<f2v *>(gvt + sidx));
+                        }
+                    } else {
+                        if (ok0) gvt[sidx] = (VT)g0;
+                        if (ok1) gvt[sidx + 1] = (VT)g1;
+                    }
+                }
 #pragma unroll
                 for (int j = 0; j < KC; j++) {
                     if (H1) {

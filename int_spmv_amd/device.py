@@ -77,6 +77,7 @@ SIGNATURES = [
     ("spmv_axt_gather_permuted", C.c_int,
      [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32, _u32]),
     ("spmv_axt_grad_vals", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("spmv_axt_gather_grad", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_axt_grad_vals_embedded", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_permute_vals", C.c_int, [_vp, _u32, C.c_int, _vp, _vp, _vp]),
     ("spmv_axt_refresh_x", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -499,6 +500,16 @@ class AxtAdjoint:
             self.eng.h, g.plan.h, _dptr(vals), _dptr(self.permT), _dptr(g.acol), _dptr(g.sec),
             _dptr(ybar), k, _dptr(xbar), k, k, 0), "spmv_axt_gather_permuted")
 
+
+    def adjoint_grad(self, ybar, xbar, x, vbar_t, vals_t=None) -> None:
+        """X_bar = A^T Y_bar (one right-hand side) with A^T's copy of the values
+        (vals_t, or the one refresh() last wrote) and, in the same pass,
+        A's value gradient in A^T's slot order (vbar_t) for the forward's x."""
+        g = self.gT
+        self.eng._check(lib().spmv_axt_gather_grad(
+            self.eng.h, g.plan.h, _dptr(vals_t if vals_t is not None else g.aval), _dptr(g.acol),
+            _dptr(g.sec), _dptr(ybar), _dptr(xbar), _dptr(x), _dptr(vbar_t)),
+            "spmv_axt_gather_grad")
 
     def tune_values_path(self, k: int = 1, reps: int = 10) -> dict:
         """Times adjoint(vals=...) by both paths on this matrix (A's current

@@ -361,3 +361,68 @@ def test_wide_passes_unaligned_rows(engine, mats, vt):
         Yr = O.csr_mm(m.row, m.col, vals, X)
         tol = 1e-12 if vt == D.F64 else 1e-6
         assert O.scaled_err_mm(m.row, m.col, vals, X, Yr, ya)[0] <= tol, (vt, k)
+
+
+@pytest.mark.parametrize("vt", [D.F64, D.F32])
+def test_adjoint_grad_fused(engine, mats, vt):
+    """spmv_axt_gather_grad: x_bar with the bits of the plain adjoint, and the
+    value gradient in A^T's slot order with the bits of spmv_axt_grad_vals
+    moved there by the slot map (padding 0)."""
+    import torch
+    for name, m in mats:
+        n = m.nrows
+        dt = np.float64 if vt == D.F64 else np.float32
+        for th in (1, 4):
+            a = H.AXT(m, np.zeros(n), th=th, want_acol=True)
+            fwd = D.DeviceAXTGather(engine, a, vt, vt)
+            adj = D.AxtAdjoint(engine, m, fwd)
+            xd = engine.upload(H.uniform(n, seed=41).astype(dt))
+            yb = engine.upload(H.normal(n, seed=42).astype(dt))
+            x1, x2 = _nan(engine, n, dt), _nan(engine, n, dt)
+            gv = _nan(engine, a.lenAX // 2, dt)
+            gvt = _nan(engine, adj.nslots_t, dt)
+            adj.adjoint(yb, x1)
+            adj.adjoint_grad(yb, x2, xd, gvt)
+            fwd.grad_vals(xd, yb, gv)
+            want = torch.empty_like(gvt)
+            engine._check(D.lib().spmv_permute_vals(engine.h, adj.nslots_t, vt, D._dptr(adj.permT),
+                                                    D._dptr(gv), D._dptr(want)), "permute")
+            assert bitwise_equal(_host(x2, n), _host(x1, n)), (name, th)
+            assert bitwise_equal(_host(gvt, adj.nslots_t), _host(want, adj.nslots_t)), (name, th)
+
+
+@pytest.mark.parametrize("vt", [D.F64, D.F32])
+def test_autograd_pair_keeps_copies_in_step(engine, vt):
+    """Plain SGD on the pair operator's two leaves: after every step A^T's
+    copy is A's values moved by the slot map, bit for bit, and each step's
+    loss, x_bar and value gradient equal the single-copy operator's."""
+    import torch
+
+    from int_spmv_amd import autograd as AG
+    m = H.CSR.powerlaw(n=1 << 12, target_nnz=80_000, maxlen=2000, seed=7)
+    n = m.nrows
+    op = AG.AxtOperator(engine, m, th=1, val_type=vt)
+    dev, dt = engine.torch_device, op.dtype
+    v, vt_ = (t.requires_grad_(True) for t in op.initial_values_pair())
+    v1 = op.initial_values().requires_grad_(True)
+    x = torch.tensor(H.uniform(n, seed=43), device=dev, dtype=dt, requires_grad=True)
+    x1 = x.detach().clone().requires_grad_(True)
+    ybar = torch.tensor(H.normal(n, seed=44), device=dev, dtype=dt)
+    lr = 0.01
+    for step in range(3):
+        L = (AG.spmv_pair(v, vt_, x, op) * ybar).sum()
+        L1 = (AG.spmv(v1, x1, op) * ybar).sum()
+        L.backward()
+        L1.backward()
+        assert bitwise_equal(L.detach().double().cpu().numpy(), L1.detach().double().cpu().numpy())
+        assert bitwise_equal(x.grad.double().cpu().numpy(), x1.grad.double().cpu().numpy()), step
+        assert bitwise_equal(v.grad.double().cpu().numpy(), v1.grad.double().cpu().numpy()), step
+        with torch.no_grad():
+            for p in (v, vt_, x, v1, x1):
+                p -= lr * p.grad
+                p.grad = None
+        want = torch.empty_like(vt_)
+        engine._check(D.lib().spmv_permute_vals(engine.h, op.adj.nslots_t, vt, D._dptr(op.adj.permT),
+                                                D._dptr(v.detach()), D._dptr(want)), "permute")
+        torch.cuda.synchronize()
+        assert bitwise_equal(vt_.detach().double().cpu().numpy(), want.double().cpu().numpy()), step
