@@ -7,7 +7,8 @@
  * `schedule(dynamic)` over rows / chunks / tiles / blocks, `omp atomic` into
  * y for the AXT kernels, 512-bit loads of the format arrays (the unaligned
  * forms, as fast as the aligned ones on aligned data), gathers of x for K1.
- * Each function names the reference function it follows.  Numerical conventions are the oracle's (oracle.h): the lane-wise
+ * Each function names the reference function it follows.  Numerical
+ * conventions are the oracle's (oracle.h): the lane-wise
  * chains are fused multiply-adds, reductions are _mm512_reduce_add_pd (the
  * halving tree), cf_CSR is one fused multiply-add per entry.
  *
