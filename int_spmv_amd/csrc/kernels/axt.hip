@@ -1114,8 +1114,11 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
                         : (fl32 || planar32) ? red_umax(th) : AXT_UMAX;
     uint32_t long_units = th == 1 ? (small ? ISPV_H1_F64_LONG : ISPV_H1_LONG)
                                   : (512 / th < 4 ? 4 : (512 / th > umax / 2 ? umax / 2 : 512 / th));
-    int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, umax, AXT_RMAX, long_units,
-                             LONG_SLOTS / ths * (th == 1 ? 1 : 8), plan);
+    const uint32_t long_max = th > 1 ? LONG_SLOTS / th
+                            : D.layout == INTSPMV_AXT_COLMAP ? GATHER_H1_LONG_MAX
+                                                             : LONG_SLOTS / 8;
+    int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, umax, AXT_RMAX, long_units, long_max,
+                             plan);
     free(uptr);
     if (st) return st;
     (*plan)->axt = D;

@@ -104,9 +104,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
         const XT *xx = (const XT *)md.x;
         AT *yy = (AT *)md.y;
         GatherLds<AT, 8> &L = lds[w];
-        // PATHS = BATCH_WIDE: wide passes (normal blocks of tile height 1, see
-        // gather_block_wide), keeping a round in the wave's narrow-pass value
-        // area; BATCH_NARROW: narrow passes of at most 8 right-hand sides
+        // PATHS = BATCH_WIDE: wide passes (tile height 1: normal blocks as
+        // gather_block_wide, keeping a round in the wave's narrow-pass value
+        // area; long rows as gather_long_wide, one tile per group in flight so
+        // that they add no registers -- the same bits as more); BATCH_NARROW:
+        // narrow passes of at most 8 right-hand sides
         static_assert(sizeof(WideLds<AT, 32, XT>) <= sizeof(L.val) &&
                           sizeof(WideLds<AT, 8, XT>) <= sizeof(L.val),
                       "a wide round fits the wave's LDS");
@@ -114,7 +116,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) void batch_kernel(
             const uint32_t kv = md.k - item.j0 < md.kc ? md.k - item.j0 : md.kc;
 #define ISPV_BW(KW)                                                                             \
     if (md.kc == KW) {                                                                          \
-        if (md.xv)                                                                              \
+        if (d.k != 0) {                                                                         \
+            if (md.xv)                                                                          \
+                gather_long_wide<VT, XT, AT, KW, true, 1>(d, av, md.acol, xx, md.ldx, yy, md.ldy,  \
+                                                       item.j0, kv, md.nrows, part);            \
+            else                                                                                \
+                gather_long_wide<VT, XT, AT, KW, false, 1>(d, av, md.acol, xx, md.ldx, yy, md.ldy, \
+                                                        item.j0, kv, md.nrows, part);           \
+        } else if (md.xv)                                                                       \
             gather_block_wide<VT, XT, AT, KW, true>(d, av, md.acol, md.sec, xx, md.ldx, yy,     \
                                                     md.ldy, item.j0, kv, md.nrows, L.key,       \
                                                     L.seg,                                      \
@@ -204,8 +213,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
             (M.nrows && (!M.con_host || !M.x || !M.y)))
             return INTSPMV_ERR_ARG;
         // kc 1, 2, 4, 8: narrow passes; at tile height 1, kc >= 2 * (16 / sizeof
-        // x) up to 32: wide passes (long rows in narrow passes of 8); 0: as
-        // spmv_axt_gather chooses
+        // x) up to 32: wide passes; 0: as spmv_axt_gather chooses
         const uint32_t vx = x_type == INTSPMV_F64 ? 2 : 4;
         const uint32_t wmin = x_type == INTSPMV_F64 ? wide_min_kc<double>() : wide_min_kc<float>();
         uint32_t kc = M.kc;
@@ -235,7 +243,8 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
         std::vector<SplitDesc> ms;
         uint32_t nlw = 0, nlong = 0, npieces = 0;
         int st = ispv_build_blocks(M.nrows, uptr.data(), umax, BLOCK_ROWS_MAX, long_units,
-                                   th == 1 ? 1024 : 8192 / th, mb, ms, nlw, nlong, npieces);
+                                   th == 1 ? GATHER_H1_LONG_MAX : 8192 / th, mb, ms, nlw, nlong,
+                                   npieces);
         if (st) return st;
         const uint32_t xv = M.ldx % vx == 0 && (uintptr_t)M.x % 16 == 0;
         hm[i] = {M.aval, M.acol, M.sec, M.x, M.y, M.nrows, th, M.k, M.ldx, M.ldy, kc, piece_base, xv};
@@ -245,15 +254,12 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
             blocks.push_back(d);
             if (d.rbeg == 0xFFFFFFFFu) continue; // XCD padding of the per-matrix plan
             const uint64_t slots = (uint64_t)(d.uend - d.ubeg) * (th == 1 ? 8 : th);
-            // long rows: narrow passes of min(kc, 8), as the kernel's kcn
-            const uint32_t step = (wide && d.k != 0 && kc > 8) ? 8 : kc;
-            for (uint32_t j0 = 0; j0 < M.k; j0 += step) {
+            for (uint32_t j0 = 0; j0 < M.k; j0 += kc) {
                 // modelled bytes of the item (as the autotuner's model): the
                 // stream plus the cache lines of its x gathers
-                const uint32_t kv = step < M.k - j0 ? step : M.k - j0;
+                const uint32_t kv = kc < M.k - j0 ? kc : M.k - j0;
                 const uint64_t xs = x_type == INTSPMV_F64 ? 8 : 4;
-                const bool wi = wide && d.k == 0;
-                const uint64_t xline = wi ? (kv * xs + 63) / 64 * 64 : (kv * xs + 15) / 16 * 64;
+                const uint64_t xline = wide ? (kv * xs + 63) / 64 * 64 : (kv * xs + 15) / 16 * 64;
                 cand.push_back({{i, bi, j0, 0}, slots * (8 + xline) + 256});
             }
         }
@@ -261,7 +267,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
         piece_base += npieces;
     }
     // Items stay in (matrix, block, first RHS) order: the narrow passes first,
-    // then the wide ones (normal blocks of tile height 1 with a wide kc), each
+    // then the wide ones (tile height 1 with a wide kc), each
     // list cut into contiguous per-XCD ranges of equal cost.
     std::vector<BatchItem> items;
     std::vector<uint32_t> xcd_range(2 * (NUM_XCD + 1));
@@ -274,7 +280,7 @@ extern "C" int intspmv_batch_create(intspmv_ctx *ctx, int val_type, int x_type, 
             const BatchMat &md = hm[c.item.mat];
             const uint32_t wmin =
                 x_type == INTSPMV_F64 ? wide_min_kc<double>() : wide_min_kc<float>();
-            const bool w = md.th == 1 && blocks[c.item.block].k == 0 && md.kc >= wmin;
+            const bool w = md.th == 1 && md.kc >= wmin;
             if (w == (pass == 1)) {
                 sel.push_back(&c);
                 cost_all += c.cost;

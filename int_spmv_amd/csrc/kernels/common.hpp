@@ -43,6 +43,16 @@ struct BlockDesc {
 // Partial sums of split pieces: PARTIAL_STRIDE slots per piece, one per
 // right-hand side (multi-RHS products use up to 32).
 constexpr uint32_t PARTIAL_STRIDE = 32;
+// Column-map plans at tile height 1 (single matrix and batch alike, so both
+// give the same bits): rows longer than this many tiles are split into pieces
+// of one wave each.  Measured on MI355X (FP64 k = 1 / k = 8, FP32 k = 32, us):
+// power-law 2^20 rows 221 / 449 / 781 at 1024, 221 / 441 / 747 at 256,
+// 205 / 429 / 731 at 128; R-MAT 2^22 rows 490 / 1066 / 2121 at 1024,
+// 515 / 1127 / 2242 at 256, 529 / 1165 / 2291 at 128.
+#ifndef ISPV_GATHER_H1_LONG_MAX
+#define ISPV_GATHER_H1_LONG_MAX 1024
+#endif
+constexpr uint32_t GATHER_H1_LONG_MAX = ISPV_GATHER_H1_LONG_MAX;
 
 // Maps the flat workgroup id so that XCD x (which receives ids x, x+8, ...)
 // gets a contiguous range [x*q + min(x, r), ...) of logical workgroups.

@@ -44,22 +44,29 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_kernel(
         vperm, xg, gvt);
 }
 
-// Wide passes (tile height 1): the normal blocks of the plan only (the
-// launch passes the normal section of the block list); long rows keep the
-// narrow passes.  See gather_block_wide.
-template <typename VT, typename XT, typename AT, int KW, bool XV>
+// Wide passes (tile height 1) over the whole plan: the long-row blocks
+// (workgroups [0, nlong_wg), one wave each, see gather_long_wide, UL tiles
+// per group in flight) and the normal blocks (see gather_block_wide), in one
+// launch as the narrow passes.
+template <typename VT, typename XT, typename AT, int KW, bool XV, int UL>
 __global__ __launch_bounds__(BLOCK_THREADS) void axt_gather_wide_kernel(
-    const BlockDesc *__restrict__ blocks, uint32_t nblocks, const VT *__restrict__ aval,
-    const uint32_t *__restrict__ acol, const uint32_t *__restrict__ sec,
-    const XT *__restrict__ x, uint32_t ldx, AT *__restrict__ y, uint32_t ldy, uint32_t j0,
-    uint32_t kv, uint32_t nrows)
+    const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
+    const VT *__restrict__ aval, const uint32_t *__restrict__ acol,
+    const uint32_t *__restrict__ sec, const XT *__restrict__ x, uint32_t ldx,
+    AT *__restrict__ y, uint32_t ldy, uint32_t j0, uint32_t kv, uint32_t nrows,
+    double *partial)
 {
     __shared__ uint32_t s_key[WAVES][G_UMAX];
     __shared__ uint16_t s_seg[WAVES][G_UMAX + 1];
     __shared__ WideLds<AT, KW, XT> s_wide[WAVES];
-    const uint32_t bid = block_index(0);
+    const uint32_t bid = block_index(nlong_wg);
     if (bid >= nblocks) return;
     const BlockDesc d = blocks[bid];
+    if (blockIdx.x < nlong_wg) {
+        gather_long_wide<VT, XT, AT, KW, XV, UL>(d, aval, acol, x, ldx, y, ldy, j0, kv, nrows,
+                                                 partial);
+        return;
+    }
     const uint32_t w = threadIdx.x / WAVE;
     gather_block_wide<VT, XT, AT, KW, XV>(d, aval, acol, sec, x, ldx, y, ldy, j0, kv, nrows,
                                           s_key[w], s_seg[w], s_wide[w]);
@@ -219,7 +226,7 @@ static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t
     const uint32_t nb = p->nblocks, th = p->axt.tileH;
     if (nb == 0) return INTSPMV_OK;
     hipStream_t s = p->ctx->stream;
-    const uint32_t nl = p->nlong_wg, nlb = nl * WAVES; // long section: blocks [0, nlb)
+    const uint32_t nl = p->nlong_wg; // long section: blocks [0, nl * WAVES)
     constexpr uint32_t V = 16 / sizeof(XT);
     const bool wide_ok = th == 1 && !PV;
     // 16-byte loads of x rows: rows and pass starts aligned to V values
@@ -235,39 +242,40 @@ static int launch_gather(const intspmv_plan *p, const void *aval, const uint32_t
         }
         const uint32_t kv = rem < kc ? rem : kc;
         if (wide_ok && kc >= wide_min_kc<XT>()) {
-            // normal blocks: one wide pass; long rows: narrow passes of min(kc, 8)
-            if (nb > nlb) {
-                const uint32_t nn = nb - nlb;
-                dim3 grid((nn + WAVES - 1) / WAVES), blk(BLOCK_THREADS);
+            // one wide pass over the long rows and the normal blocks.  FP32 /
+            // bf16: a plan without long rows takes the kernel with one long-row
+            // tile per group in flight, whose registers leave more waves per
+            // SIMD (the long-row path sets them: FP32 stencil 128^3 k = 32
+            // 617 us with one against 647 us with two); FP64 needs as many
+            // registers for the normal blocks, so it always takes two.
+            dim3 grid(ispv_grid(p)), blk(BLOCK_THREADS);
+#define ISPV_GWU(KW, XVB, ULV)                                                                  \
+    axt_gather_wide_kernel<VT, XT, AT, KW, XVB, ULV><<<grid, blk, 0, s>>>(                      \
+        p->d_blocks, nl, nb, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y, ldy, j0, \
+        kv, p->nrows, p->d_partial)
 #define ISPV_GW(KW)                                                                             \
     if (kc == KW) {                                                                             \
-        if (xv)                                                                                 \
-            axt_gather_wide_kernel<VT, XT, AT, KW, true><<<grid, blk, 0, s>>>(                  \
-                p->d_blocks + nlb, nn, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y, \
-                ldy, j0, kv, p->nrows);                                                         \
-        else                                                                                    \
-            axt_gather_wide_kernel<VT, XT, AT, KW, false><<<grid, blk, 0, s>>>(                 \
-                p->d_blocks + nlb, nn, (const VT *)aval, acol, sec, (const XT *)x, ldx, (AT *)y, \
-                ldy, j0, kv, p->nrows);                                                         \
+        if constexpr (sizeof(XT) == 8) {                                                        \
+            if (xv) ISPV_GWU(KW, true, wide_long_u<XT>());                                      \
+            else ISPV_GWU(KW, false, wide_long_u<XT>());                                        \
+        } else if (nl) {                                                                        \
+            if (xv) ISPV_GWU(KW, true, wide_long_u<XT>());                                      \
+            else ISPV_GWU(KW, false, wide_long_u<XT>());                                        \
+        } else {                                                                                \
+            if (xv) ISPV_GWU(KW, true, 1);                                                      \
+            else ISPV_GWU(KW, false, 1);                                                        \
+        }                                                                                       \
     }
-                if constexpr (V == 2) {
-                    ISPV_GW(4)
-                }
-                ISPV_GW(8)
-                ISPV_GW(16)
-                ISPV_GW(32)
+            if constexpr (V == 2) {
+                ISPV_GW(4)
+            }
+            ISPV_GW(8)
+            ISPV_GW(16)
+            ISPV_GW(32)
 #undef ISPV_GW
-                int st = ispv_launch_check(p->ctx);
-                if (st) return st;
-            }
-            const uint32_t kl = kc > 8 ? 8 : kc; // as the batch kernel's long rows
-            for (uint32_t jj = 0; nl && jj < kv; jj += kl) {
-                launch_narrow<VT, XT, AT, PV>(p, p->d_blocks, nl, nlb, dim3(nl), aval, acol, sec,
-                                              x, ldx, y, ldy, j0 + jj, kl,
-                                              kv - jj < kl ? kv - jj : kl, vperm);
-                int st = ispv_launch_check(p->ctx);
-                if (st) return st;
-            }
+#undef ISPV_GWU
+            int st = ispv_launch_check(p->ctx);
+            if (st) return st;
         } else {
             launch_narrow<VT, XT, AT, PV>(p, p->d_blocks, nl, nb, dim3(ispv_grid(p)), aval, acol,
                                           sec, x, ldx, y, ldy, j0, kc, kv, vperm);

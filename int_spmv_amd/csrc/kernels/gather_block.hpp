@@ -543,4 +543,81 @@ __device__ __forceinline__ void gather_block_wide(const BlockDesc &d, const VT *
     });
 }
 
+// Tiles per group in flight in a wide pass over a long row (the tile order of
+// each lane's sums does not depend on it).  Measured with the normal blocks in
+// the same kernel (us, power-law 2^20 rows FP32 k = 32 / R-MAT 2^22 rows FP32
+// k = 8): 518.5 / 799.0 with one, 500.0 / 784.5 with two; FP64 unchanged.
+#ifndef ISPV_WIDE_LONG_U
+#define ISPV_WIDE_LONG_U 2
+#endif
+template <typename XT>
+constexpr int wide_long_u() { return ISPV_WIDE_LONG_U; }
+
+// A wide pass over a long-row block (d.k >= 1, tile height 1: every tile of
+// the block is the row's, so no keys are read).  Group grp takes tiles
+// grp, grp + NG, grp + 2 NG, ... in that order and each lane keeps a running
+// sum per right-hand side of the tiles' tree sums (FP32: compensated); the NG
+// groups' sums then meet in a butterfly over the lane bits above the group,
+// and group 0 writes y (d.k == 1) or the piece's partial sums (d.k >= 2).
+template <typename VT, typename XT, typename AT, int KW, bool XV, int UL = wide_long_u<XT>()>
+__device__ __forceinline__ void gather_long_wide(const BlockDesc &d, const VT *__restrict__ aval,
+                                                 const uint32_t *__restrict__ acol,
+                                                 const XT *__restrict__ x, uint32_t ldx,
+                                                 AT *__restrict__ y, uint32_t ldy, uint32_t j0,
+                                                 uint32_t kv, uint32_t nrows, double *partial)
+{
+    using S = WideShape<KW, XT>;
+    constexpr int V = S::V, G = S::G, NG = S::NG;
+    const int lane = lane_id();
+    const uint32_t grp = lane / G, jl = (lane % G) * V;
+    const int cnt = (int)kv - (int)jl; // right-hand sides left for this lane (<= 0: none)
+    const uint32_t nu = d.uend - d.ubeg;
+    if (nu == 0) return; // XCD padding of the plan's long section
+    LaneSum<AT> acc[V];
+    for (uint32_t b0 = 0; b0 < nu; b0 += UL * NG) {
+        AT ts[UL][V];
+#pragma unroll
+        for (int u = 0; u < UL; u++) {
+            const uint32_t t = b0 + grp + NG * u;
+            const size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
+            AT v[8];
+            tile_values<VT, AT>(aval, T, v);
+            typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+            const u4v c0 = ntload(reinterpret_cast<const u4v *>(acol + T * 8));
+            const u4v c1 = ntload(reinterpret_cast<const u4v *>(acol + T * 8 + 4));
+            AT xv[8][V];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                wide_x<XT, AT, V, XV>(x, c0[q], ldx, j0 + jl, cnt, xv[q]);
+                wide_x<XT, AT, V, XV>(x, c1[q], ldx, j0 + jl, cnt, xv[q + 4]);
+            }
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                AT pr[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) pr[q] = v[q] * xv[q][e];
+                ts[u][e] = ((pr[0] + pr[4]) + (pr[2] + pr[6])) + ((pr[1] + pr[5]) + (pr[3] + pr[7]));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UL; u++)
+            if (b0 + grp + NG * u < nu)
+#pragma unroll
+                for (int e = 0; e < V; e++) acc[e].add(ts[u][e]);
+    }
+#pragma unroll
+    for (int e = 0; e < V; e++) {
+        AT s = acc[e].value();
+#pragma unroll
+        for (int m = G; m < WAVE; m <<= 1) s += __shfl_xor(s, m, WAVE);
+        if (grp == 0 && e < cnt) {
+            if (d.k == 1) {
+                if (d.rbeg < nrows) y[(size_t)d.rbeg * ldy + j0 + jl + e] = s;
+            } else {
+                partial[(size_t)d.piece * PARTIAL_STRIDE + j0 + jl + e] = (double)s;
+            }
+        }
+    }
+}
+
 } // namespace ispv

@@ -106,7 +106,9 @@ def test_gather_wide_x_plain_loads(engine, mats, vt):
 def test_gather_sixteen_per_pass(engine, mats, vt):
     """Passes of 16 right-hand sides (narrow at tile height 4, wide at tile
     height 1) against passes of 8: each right-hand side keeps its own sums in
-    the same order, so the bits are equal; FP64 refuses narrow passes of 16."""
+    the same order, so the bits are equal -- at tile height 1 for the rows of
+    the plan's normal blocks (a wide pass adds a long row's tiles in another
+    order: within tolerance); FP64 refuses narrow passes of 16."""
     import torch
     for name, m in mats[:2]:
         n = m.nrows
@@ -119,9 +121,15 @@ def test_gather_sixteen_per_pass(engine, mats, vt):
             g.matvec(Xd, Y16, k=k)
             g.matvec(Xd, Y8, k=k, kc=8)
             torch.cuda.synchronize()
-            assert bitwise_equal(Y16.cpu().numpy(), Y8.cpu().numpy()), (name, th, k)
+            y16, y8 = Y16.cpu().numpy().reshape(n, k), Y8.cpu().numpy().reshape(n, k)
+            ok = (np.diff(m.row.astype(np.int64)) + 7) // 8 <= 64 if th == 1 else slice(None)
+            assert bitwise_equal(y16[ok], y8[ok]), (name, th, k)
             Yg = _host(Y16, n * k).reshape(n, k)
+            Y8g = _host(Y8, n * k).reshape(n, k)
             for j in (0, k - 1):
+                yr = O.cf_csr(m.row, m.col, m.val, X[:, j].astype(np.float64))
+                tol = 1e-6 if vt == D.F32 else 1e-2
+                assert scaled(m, X[:, j].astype(np.float64), yr, Y8g[:, j]) <= tol, (name, th, k, j)
                 yr = O.cf_csr(m.row, m.col, m.val, X[:, j].astype(np.float64))
                 tol = 1e-6 if vt == D.F32 else 1e-2
                 assert scaled(m, X[:, j].astype(np.float64), yr, Yg[:, j]) <= tol, (name, th, k, j)

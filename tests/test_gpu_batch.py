@@ -57,6 +57,36 @@ def test_batch_f64_equals_single(engine):
             assert scaled(m, e["Xh"][:, j], yr, Yb[:, j]) <= 1e-12
 
 
+@pytest.mark.parametrize("vt,ks,tol", [(D.F64, (8, 5, 4), 1e-12), (D.F32, (32, 13, 8), 1e-6)])
+def test_batch_wide_long_rows_equal_single(engine, vt, ks, tol):
+    """Wide passes over long rows, split rows among them (tile height 1, one
+    pass of 4..32 right-hand sides, lanes past k idle): the batch holds the
+    single-matrix launch's bits, FP32 included (the batch keeps one tile per
+    group of lanes in flight, the single-matrix kernel two for FP64: the same
+    order), and both are within tolerance of cf_CSR in every column."""
+    import torch
+    dt = torch.float64 if vt == D.F64 else torch.float32
+    mats = [(csr_long_rows(), ks[0]), (csr_long_rows(n=9000, seed=6), ks[1]),
+            (H.CSR.powerlaw(n=1 << 13, target_nnz=200_000, maxlen=4096, seed=2), ks[2]),
+            (csr_sparse_gaps(), ks[0])]
+    ents = _entries(engine, mats, (1,), dt)
+    b = D.DeviceBatch(engine, ents, vt, vt)
+    b.launch()
+    assert b.info()["nsplit"] > 0
+    for i, e in enumerate(ents):
+        print(f"[wide long rows] matrix {i} n={e['m'].nrows} k={e['k']}", flush=True)
+        g = D.DeviceAXTGather(engine, e["a"], vt, vt)
+        Y1 = torch.full_like(e["Y"], float("nan"))
+        g.matvec(e["X"], Y1, k=e["k"])
+        torch.cuda.synchronize()
+        Yb = _y(e)
+        assert bitwise_equal(Yb.reshape(-1), Y1.double().cpu().numpy()), i
+        m = e["m"]
+        for j in range(e["k"]):
+            yr = O.cf_csr(m.row, m.col, m.val, e["Xh"][:, j])
+            assert scaled(m, e["Xh"][:, j], yr, Yb[:, j]) <= tol, (i, j)
+
+
 def test_batch_f32_and_graph_replay(engine):
     import torch
     mats = batch_256(seed=2, count=24, nmax=30000)

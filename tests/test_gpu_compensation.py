@@ -7,8 +7,9 @@ CSR order loses every s (1 + 2^-25 rounds back to 1, below half an ulp) and
 ends at 0: scaled error (L - 2) s / (2 + (L - 2) s) > 1e-4.  A compensated
 running sum keeps them.  Every value here is exact in FP32 and BF16, so the
 storage type adds no error and each FP32 kernel path (embedded FP32 and
-BF16 storage at several tile heights, COM, the column map with 1 and 3
-right-hand sides, the batched launch) must meet 1e-6 on these rows.
+BF16 storage at several tile heights, COM, the column map with 1, 3 and 16
+right-hand sides -- narrow and wide passes -- and the batched launch) must
+meet 1e-6 on these rows.
 """
 import numpy as np
 import pytest
@@ -112,17 +113,19 @@ def test_fp32_long_rows_compensated(engine):
     for th in (1, 4):
         a = H.AXT(m, np.zeros(n), th=th, want_acol=True)
         g = D.DeviceAXTGather(engine, a, D.F32, D.F32)
-        for k in (1, 3):
+        for k in (1, 3, 16):  # 16 at tile height 1: a wide pass
             X = x32.repeat_interleave(k)  # n x k row-major, every column x
             Y = nan(n * k)
             g.matvec(X, Y, k=k)
             Yh = host(Y, n * k).reshape(n, k)
             for j in range(k):
                 _check(m, x, Yh[:, j], 1e-6, f"column map th{th} k{k} col{j}")
-        Y = nan(n * 3)
-        b = D.DeviceBatch(engine, [dict(a=a, X=x32.repeat_interleave(3), Y=Y, k=3)], D.F32, D.F32)
-        b.launch()
-        Yh = host(Y, n * 3).reshape(n, 3)
-        for j in range(3):
-            _check(m, x, Yh[:, j], 1e-6, f"batch th{th} col{j}")
-        del b
+        for k in (3, 16):
+            Y = nan(n * k)
+            b = D.DeviceBatch(engine, [dict(a=a, X=x32.repeat_interleave(k), Y=Y, k=k)], D.F32,
+                              D.F32)
+            b.launch()
+            Yh = host(Y, n * k).reshape(n, k)
+            for j in range(k):
+                _check(m, x, Yh[:, j], 1e-6, f"batch th{th} k{k} col{j}")
+            del b
