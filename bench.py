@@ -126,7 +126,7 @@ def bench_batch(eng, count, torch, log, tune="model"):
     banded), FP32 values and x, k in {1,4,8,16,32} each; per-matrix tile
     height, right-hand-side blocking and units per wave from the autotuner
     (tune="device": byte model + device timing; "model": byte model only, 128
-    units per wave); one persistent launch captured in a hipGraph and
+    units per wave); one launch (two concurrent persistent kernels) captured in a hipGraph and
     replayed."""
     import collections
 

@@ -165,11 +165,11 @@ int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval
  * narrow for F32/BF16 values at any tile height; with tile height 1,
  * kc >= 2 * (16 / sizeof x) -- 8, 16, 32 for F32 x; 4, 8, 16, 32 for F64 x --
  * is a wide pass: lanes along the right-hand sides, a group of lanes reading
- * each x row of kc values together, the matrix streamed once per pass (rows
- * longer than a block take narrow passes of 8).  Both give the same bits for
- * rows inside a block.  0 = automatic: at tile height 1 one wide pass of the
- * smallest width >= what is left of k (up to 32), else the largest narrow
- * width <= it. */
+ * each x row of kc values together, the matrix streamed once per pass, long
+ * rows included.  Both give the same bits for rows inside a block; long rows
+ * (more than a block) agree within rounding.  0 = automatic: at tile height 1
+ * one wide pass of the smallest width >= what is left of k (up to 32), else
+ * the largest narrow width <= it. */
 /* The same product with the slot values read through a permutation: value
  * of slot s = vals[vperm[s]] (0 where vperm[s] = 0xFFFFFFFF; vperm 8-byte
  * aligned).  On the transposed tile map, with its slot map (A^T slot -> A
@@ -216,10 +216,11 @@ int spmv_axt_refresh_x(intspmv_ctx *ctx, const intspmv_plan *plan, const uint32_
  * Many AXT NOC matrices in column-map form, each with its own tile height,
  * 1..32 right-hand sides and right-hand-side blocking kc (1, 2, 4, 8: narrow
  * passes; at tile height 1 also the wide passes of spmv_axt_gather, kc >=
- * 2 * (16 / sizeof x) up to 32; 0 = as spmv_axt_gather chooses), run by ONE
- * persistent kernel (plus one
- * fix-up kernel when some row is longer than a wave streams alone).  The work
- * list is built once on the host.  intspmv_batch_graph_capture records the
+ * 2 * (16 / sizeof x) up to 32; 0 = as spmv_axt_gather chooses), run by two
+ * persistent kernels -- narrow and wide passes, on the caller's stream and a
+ * second stream of the batch, joined back -- plus one fix-up kernel when some
+ * row is longer than a wave streams alone.  A matrix's rows get the bits of
+ * spmv_axt_gather with the same kc.  The work list is built once on the host.  intspmv_batch_graph_capture records the
  * launch in a hipGraph once; intspmv_batch_graph_launch replays it.  Pointers
  * in intspmv_batch_matrix are device pointers except con_host.  All matrices of
  * a batch share val_type / x_type (F64/F64, F32/F32 or BF16/F32).  No

@@ -572,7 +572,8 @@ def upload_batch_arrays(eng: Engine, a, val_type: int):
 
 class DeviceBatch:
     """Many column-map AXT matrices with their own tile heights and 1..32
-    right-hand sides in one persistent launch (optionally a hipGraph).
+    right-hand sides in one launch of two concurrent persistent kernels (narrow
+    and wide passes; optionally captured in a hipGraph).
 
     entries: list of dicts with keys a (host AXT built with want_acol), X and
     Y (device tensors, n*k row-major), k and optionally ldx (leading
