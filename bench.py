@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--gather-only", type=int, default=0, metavar="K",
                    help="only the FP64 column-map product with K right-hand sides on the workload's "
                         "matrix, tile height 1 (profiling)")
+    p.add_argument("--placement", choices=("tuned", "plain", "xcd"), default="tuned",
+                   help="--gather-only: the column-map plan's placement (tuned: the faster)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--batch-count", type=int, default=256,
                    help="matrices in the batched-launch measurement (0 = skip)")
@@ -438,12 +440,17 @@ def main():
         k = args.gather_only
         ag = H.AXT(m, x, th=1, want_acol=True)
         g = D.DeviceAXTGather(eng, ag)
+        if args.placement == "tuned":
+            g.tune_placement(k=k)
+        else:
+            g.set_placement(args.placement)
         X = torch.rand(n * k, dtype=torch.float64, device=eng.torch_device)
         Y = eng.empty(n * k)
         w, e = timed(lambda: g.matvec(X, Y, k=k), args.steps, args.warmup, torch)
         b = M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, k=k)
         print(json.dumps({"gather": {"workload": args.workload, "k": k, "us": round(e * 1e6, 2),
-                                     "bytes": b, "GBs": round(b / e / 1e9, 1)}}), flush=True)
+                                     "placement": g.placement, "bytes": b,
+                                     "GBs": round(b / e / 1e9, 1)}}), flush=True)
         return
     xd = eng.upload(x)
     formats = {}
@@ -536,13 +543,17 @@ def main():
                 keep(nm, "y", y, n, tol=1e-6 if prec == "F32" else 1e-2)
                 del d
         # column-map form: 1 and 8 right-hand sides, adjoint on the transposed map
+        # each column-map product first picks its plan's placement on this
+        # matrix (long rows dealt over all XCDs, or row groups pinned to XCDs;
+        # same bits) by timing both, as a caller would at set-up
         ag = H.AXT(m, x, th=1, want_acol=True)
         g = D.DeviceAXTGather(eng, ag)
+        pl1 = g.tune_placement(k=1)
         y = nanv(n)
         w, e = timed(lambda: g.matvec(xd, y), 20, 3, torch)
         fwd_g = e
         record("AXT_GATHER_H1_F64_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC), w, e,
-               {"vs_embedded": round(e / head_ev, 3)})
+               {"vs_embedded": round(e / head_ev, 3), "placement": pl1})
         keep("AXT_GATHER_H1_F64_K1", "y", y, n)
         # conversion: a new x into the embedded AXT arrays, on the device
         # (spmv_axt_refresh_x) versus the host builder (getFormatDataAXT)
@@ -558,17 +569,21 @@ def main():
         k8 = 8
         X8 = torch.rand(n * k8, dtype=torch.float64, device=eng.torch_device) * 2 - 1
         Y8 = nanv(n * k8)
+        pl8 = g.tune_placement(k=k8)
         w, e = timed(lambda: g.matvec(X8, Y8, k=k8), 10, 2, torch)
         record("AXT_GATHER_H1_F64_K8", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, k=k8), w, e,
-               {"GFLOPs": round(M.flops(nnz, k8) / e / 1e9, 1)})
+               {"GFLOPs": round(M.flops(nnz, k8) / e / 1e9, 1), "placement": pl8})
+        g.set_placement(pl1["placement"])
         keep("AXT_GATHER_H1_F64_K8", "mm", Y8, n * k8, X=X8.cpu().numpy().reshape(n, k8))
         adj = D.AxtAdjoint(eng, m, g)
+        pla = adj.gT.tune_placement(k=1)
         yb = torch.randn(n, dtype=torch.float64, device=eng.torch_device)
         xb = nanv(n)
         w, e = timed(lambda: adj.adjoint(yb, xb), 20, 3, torch)
         at = adj.at
         record("AXT_ADJOINT_H1_F64", M.bytes_axt_gather(n, at.lenAX, at.lenSEC), w, e,
-               {"vs_forward_gather": round(e / fwd_g, 3), "vs_forward_embedded": round(e / head_ev, 3)})
+               {"vs_forward_gather": round(e / fwd_g, 3), "vs_forward_embedded": round(e / head_ev, 3),
+                "placement": pla})
         ybh = yb.cpu().numpy()
         keep("AXT_ADJOINT_H1_F64", "adj", xb, n, ybar=ybh)
         # right after a value update (a training step): A's values read through
@@ -613,28 +628,32 @@ def main():
         g32 = D.DeviceAXTGather(eng, ag, D.F32, D.F32)
         x32 = xd.float()
         y32 = nanv(n, np.float32)
+        pl32 = g32.tune_placement(k=1)
         w, e = timed(lambda: g32.matvec(x32, y32), 20, 3, torch)
         fwd32 = e
         record("AXT_GATHER_H1_F32_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, sv=4, sx=4, sy=4),
-               w, e)
+               w, e, {"placement": pl32})
         keep("AXT_GATHER_H1_F32_K1", "y", y32, n, tol=1e-6)
         k32 = 32  # 16 right-hand sides per pass
         X32 = torch.rand(n * k32, dtype=torch.float32, device=eng.torch_device) * 2 - 1
         Y32 = nanv(n * k32, np.float32)
+        pl32k = g32.tune_placement(k=k32)
         w, e = timed(lambda: g32.matvec(X32, Y32, k=k32), 10, 2, torch)
         record("AXT_GATHER_H1_F32_K32",
                M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC, k=k32, sv=4, sx=4, sy=4), w, e,
-               {"GFLOPs": round(M.flops(nnz, k32) / e / 1e9, 1)})
+               {"GFLOPs": round(M.flops(nnz, k32) / e / 1e9, 1), "placement": pl32k})
+        g32.set_placement(pl32["placement"])
         keep("AXT_GATHER_H1_F32_K32", "mm", Y32, n * k32,
              X=X32.double().cpu().numpy().reshape(n, k32), tol=1e-6)
         del X32, Y32
         adj32 = D.AxtAdjoint(eng, m, g32)
+        pla32 = adj32.gT.tune_placement(k=1)
         yb32 = yb.float()
         xb32 = nanv(n, np.float32)
         w, e = timed(lambda: adj32.adjoint(yb32, xb32), 20, 3, torch)
         at = adj32.at
         record("AXT_ADJOINT_H1_F32", M.bytes_axt_gather(n, at.lenAX, at.lenSEC, sv=4, sx=4, sy=4),
-               w, e, {"vs_forward_gather": round(e / fwd32, 3)})
+               w, e, {"vs_forward_gather": round(e / fwd32, 3), "placement": pla32})
         keep("AXT_ADJOINT_H1_F32", "adj", xb32, n, ybar=yb32.double().cpu().numpy(), tol=1e-6)
         del g32, adj32, ag, x32, y32, yb32, xb32
         dc = D.DeviceCSR(eng, m)

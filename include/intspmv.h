@@ -144,6 +144,19 @@ enum { INTSPMV_AXT_INTERLEAVED = 0, INTSPMV_AXT_PLANAR = 1, INTSPMV_AXT_COLMAP =
 /* con_host: str_matAXT.con (lenCON = nrows) from the builder. */
 int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc, const uint32_t *con_host,
                      intspmv_plan **plan);
+/* A column-map plan (layout INTSPMV_AXT_COLMAP) whose rows come in ngroups
+ * contiguous groups, rows [group_rows[g], group_rows[g+1]), group_rows[0] = 0,
+ * group_rows[ngroups] = nrows; group_rows = NULL cuts ngroups groups of about
+ * equal tiles.  Group g runs on XCD g % 8 of the MI355X (its work blocks, long
+ * rows included, go only to the workgroups that XCD receives), so the x
+ * values of a band around a group's rows stay in that XCD's L2.  Same
+ * products and the same bits as intspmv_axt_plan's plan; which is faster
+ * depends on the matrix (power-law rows with a diagonal band: grouped;
+ * R-MAT with multi-RHS wide passes: ungrouped).  No reference counterpart
+ * (the reference schedules rows with OpenMP, intSpmv_main.c:1391). */
+int intspmv_axt_plan_grouped(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
+                             const uint32_t *con_host, uint32_t ngroups,
+                             const uint32_t *group_rows, intspmv_plan **plan);
 /* Embedded-x product.  INTERLEAVED: ax = the reference array, xplane = NULL.
  * PLANAR: ax = value plane, xplane = x plane.  y has the accumulation type
  * (double for F64 values, float otherwise). */
@@ -275,6 +288,12 @@ int intspmv_plan_layout(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uin
                         uint32_t long_units, uint32_t long_max, uint32_t *blocks,
                         size_t cap_blocks, uint32_t *nblocks, uint32_t *splits,
                         size_t cap_splits, uint32_t *nsplits, uint32_t *nlong_wg);
+/* The same for a plan with row groups pinned to XCDs (intspmv_axt_plan_grouped). */
+int intspmv_plan_layout_grouped(uint32_t nrows, const uint32_t *uptr, uint32_t umax,
+                                uint32_t rmax, uint32_t long_units, uint32_t long_max,
+                                uint32_t ngroups, const uint32_t *group_rows, uint32_t *blocks,
+                                size_t cap_blocks, uint32_t *nblocks, uint32_t *splits,
+                                size_t cap_splits, uint32_t *nsplits, uint32_t *nlong_wg);
 /* number of work blocks / split rows of a plan (for reporting) */
 int intspmv_plan_info(const intspmv_plan *plan, uint32_t *nblocks, uint32_t *nsplit_rows,
                       uint32_t *nlong_rows);

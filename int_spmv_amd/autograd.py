@@ -40,6 +40,11 @@ class AxtOperator:
         values (through the slot map, or a refresh of A^T's copy first)."""
         return self.adj.tune_values_path()
 
+    def tune_placement(self) -> dict:
+        """Picks, for this matrix, the plan placement (row groups pinned to
+        XCDs or not; same bits) of the forward and of x_bar = A^T y_bar."""
+        return dict(forward=self.fwd.tune_placement(k=1), adjoint=self.adj.gT.tune_placement(k=1))
+
     def initial_values(self):
         """The matrix values in slot order (a leaf tensor to optimise)."""
         return self.fwd.aval.clone()

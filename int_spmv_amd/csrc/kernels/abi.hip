@@ -259,27 +259,19 @@ int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size
     return INTSPMV_OK;
 }
 
-// Packs rows into blocks.  uptr[r]..uptr[r+1] are row r's units (monotone,
-// n+1 entries).  Rows with at most long_units units are packed greedily into
-// normal blocks of at most umax units and rmax rows.  Longer rows become
-// long blocks of one wave each; rows longer than long_max are split into
-// ceil(len / long_max) near-equal pieces combined by the fix-up kernel.  The
-// long blocks come first, longest first, padded to whole XCD rounds of
-// workgroups; the normal blocks follow in row order.  Pure host code.
-int ispv_build_blocks(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
-                      uint32_t long_units, uint32_t long_max,
-                      std::vector<ispv::BlockDesc> &blocks,
-                      std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg, uint32_t &nlong,
-                      uint32_t &npieces)
+// Packs rows [r0, r1) into blocks.  uptr[r]..uptr[r+1] are row r's units
+// (monotone).  Rows with at most long_units units are packed greedily into
+// normal blocks of at most umax units and rmax rows.  Longer rows become long
+// blocks of one wave each; rows longer than long_max are split into
+// ceil(len / long_max) near-equal pieces (k = pieces, pad1 = index of the
+// piece within its row until the pieces are numbered).
+static void collect_blocks(uint32_t r0, uint32_t r1, const uint32_t *uptr, uint32_t umax,
+                           uint32_t rmax, uint32_t long_units, uint32_t long_max,
+                           std::vector<ispv::BlockDesc> &normal, std::vector<ispv::BlockDesc> &lng,
+                           uint32_t &nlong)
 {
-    if (!uptr || umax == 0 || rmax == 0 || long_units >= umax || long_max <= long_units)
-        return INTSPMV_ERR_ARG;
-    for (uint32_t r = 0; r < nrows; r++)
-        if (uptr[r + 1] < uptr[r]) return INTSPMV_ERR_ARG;
-    std::vector<ispv::BlockDesc> normal, lng;
-    nlong = 0;
-    uint32_t r = 0;
-    while (r < nrows) {
+    uint32_t r = r0;
+    while (r < r1) {
         uint32_t len = uptr[r + 1] - uptr[r];
         if (len > long_units) {
             uint32_t np = (len + long_max - 1) / long_max;
@@ -291,50 +283,165 @@ int ispv_build_blocks(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint3
                 for (uint32_t q = 0; q < np; q++) {
                     uint64_t a = uptr[r] + q * per, b = a + per;
                     if (b > uptr[r + 1]) b = uptr[r + 1];
-                    // pad1 = index of the piece within its row; b0 and piece
-                    // are filled in below once the pieces have their final place
                     lng.push_back({(uint32_t)a, (uint32_t)b, r, r + 1, 0, np, 0, q});
                 }
             }
             r++;
             continue;
         }
-        uint32_t r0 = r, units = 0;
-        while (r < nrows && r - r0 < rmax) {
+        uint32_t rs = r, units = 0;
+        while (r < r1 && r - rs < rmax) {
             uint32_t l = uptr[r + 1] - uptr[r];
             if (l > long_units || units + l > umax) break;
             units += l;
             r++;
         }
-        normal.push_back({uptr[r0], uptr[r], r0, r, 0, 0, 0, 0});
+        normal.push_back({uptr[rs], uptr[r], rs, r, 0, 0, 0, 0});
     }
-    // longest first; pieces of one row stay together and in order
+}
+
+// Longest first; the pieces of one split row stay together and in order.
+static void sort_long(std::vector<ispv::BlockDesc> &lng)
+{
     std::stable_sort(lng.begin(), lng.end(), [](const ispv::BlockDesc &a, const ispv::BlockDesc &b) {
         uint32_t la = a.k > 1 ? 0xFFFFFFFFu - a.rbeg : a.uend - a.ubeg;
         uint32_t lb = b.k > 1 ? 0xFFFFFFFFu - b.rbeg : b.uend - b.ubeg;
         return la > lb;
     });
-    splits.clear();
-    npieces = 0;
-    for (size_t i = 0; i < lng.size(); i++)
-        if (lng[i].k > 1 && lng[i].pad1 == 0) {
+}
+
+// Numbers the split pieces of a sorted long-block list in order (each
+// piece's slot in the partial-sum scratch, continuing from npieces) and
+// appends the split rows.
+static int number_pieces(std::vector<ispv::BlockDesc> &blocks, std::vector<ispv::SplitDesc> &splits,
+                         uint32_t &npieces)
+{
+    for (size_t i = 0; i < blocks.size(); i++)
+        if (blocks[i].k > 1 && blocks[i].pad1 == 0 && blocks[i].rbeg != 0xFFFFFFFFu) {
             // the k pieces of this row follow in order (stable sort, equal keys)
-            const uint32_t k = lng[i].k;
-            if (i + k > lng.size()) return INTSPMV_ERR_ARG;
-            splits.push_back({lng[i].rbeg, npieces, k, 0});
+            const uint32_t k = blocks[i].k;
+            if (i + k > blocks.size()) return INTSPMV_ERR_ARG;
+            splits.push_back({blocks[i].rbeg, npieces, k, 0});
             for (uint32_t q = 0; q < k; q++) {
-                lng[i + q].b0 = (uint32_t)i;
-                lng[i + q].piece = npieces++;
-                lng[i + q].pad1 = 0;
+                if (blocks[i + q].rbeg != blocks[i].rbeg || blocks[i + q].pad1 != q)
+                    return INTSPMV_ERR_ARG;
+                blocks[i + q].b0 = (uint32_t)i;
+                blocks[i + q].piece = npieces++;
+                blocks[i + q].pad1 = 0;
             }
             i += k - 1;
         }
+    return INTSPMV_OK;
+}
+
+static const ispv::BlockDesc LONG_FILL = {0, 0, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 1, 0, 0};
+
+static bool blocks_args_ok(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
+                           uint32_t long_units, uint32_t long_max)
+{
+    if (!uptr || umax == 0 || rmax == 0 || long_units >= umax || long_max <= long_units)
+        return false;
+    for (uint32_t r = 0; r < nrows; r++)
+        if (uptr[r + 1] < uptr[r]) return false;
+    return true;
+}
+
+// Packs all rows into blocks (collect_blocks).  The long blocks come first,
+// longest first, padded to whole XCD rounds of workgroups; the normal blocks
+// follow in row order.  Pure host code.
+int ispv_build_blocks(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
+                      uint32_t long_units, uint32_t long_max,
+                      std::vector<ispv::BlockDesc> &blocks,
+                      std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg, uint32_t &nlong,
+                      uint32_t &npieces)
+{
+    if (!blocks_args_ok(nrows, uptr, umax, rmax, long_units, long_max)) return INTSPMV_ERR_ARG;
+    std::vector<ispv::BlockDesc> normal, lng;
+    nlong = 0;
+    collect_blocks(0, nrows, uptr, umax, rmax, long_units, long_max, normal, lng, nlong);
+    sort_long(lng);
     const uint32_t per_round = ispv::WAVES * ispv::NUM_XCD;
     size_t nl = lng.empty() ? 0 : (lng.size() + per_round - 1) / per_round * per_round;
-    while (lng.size() < nl) lng.push_back({0, 0, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 1, 0, 0});
+    while (lng.size() < nl) lng.push_back(LONG_FILL);
+    splits.clear();
+    npieces = 0;
+    int st = number_pieces(lng, splits, npieces);
+    if (st) return st;
     blocks = lng;
     blocks.insert(blocks.end(), normal.begin(), normal.end());
     nlong_wg = (uint32_t)(nl / ispv::WAVES);
+    return INTSPMV_OK;
+}
+
+// Blocks of rows in groups pinned to XCDs.  Group g is rows
+// [gbeg[g], gbeg[g+1]) and runs on XCD g % 8: blocks never straddle two
+// groups, and the block list is laid out for block_index() so that
+// workgroups on XCD x (hardware ids x, x+8, ...) take only blocks of its
+// groups.  Long section: round r's workgroup 8r + x holds the next WAVES
+// long blocks of XCD x (longest first over its groups); normal section: XCD
+// x's normal blocks (its groups in order) padded with empty blocks to the
+// longest XCD list, so the contiguous range block_index() hands XCD x is
+// exactly its own list.  gbeg == NULL: ngroups groups of about equal units.
+// Used where the x values a group of rows gathers should stay in one XCD's
+// L2 (banded matrices whose long rows would otherwise land on any XCD).
+int ispv_build_blocks_grouped(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
+                              uint32_t long_units, uint32_t long_max, uint32_t ngroups,
+                              const uint32_t *gbeg, std::vector<ispv::BlockDesc> &blocks,
+                              std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg,
+                              uint32_t &nlong, uint32_t &npieces)
+{
+    using ispv::NUM_XCD;
+    using ispv::WAVES;
+    if (!blocks_args_ok(nrows, uptr, umax, rmax, long_units, long_max) || ngroups == 0)
+        return INTSPMV_ERR_ARG;
+    std::vector<uint32_t> even;
+    if (!gbeg) {
+        // ngroups groups of about equal units, cut at row starts
+        even.resize(ngroups + 1);
+        const uint64_t tot = uptr[nrows];
+        uint32_t r = 0;
+        even[0] = 0;
+        for (uint32_t g = 1; g < ngroups; g++) {
+            const uint64_t want = tot * g / ngroups;
+            while (r < nrows && uptr[r] < want) r++;
+            even[g] = r;
+        }
+        even[ngroups] = nrows;
+        gbeg = even.data();
+    }
+    if (gbeg[0] != 0 || gbeg[ngroups] != nrows) return INTSPMV_ERR_ARG;
+    for (uint32_t g = 0; g < ngroups; g++)
+        if (gbeg[g + 1] < gbeg[g]) return INTSPMV_ERR_ARG;
+    std::vector<ispv::BlockDesc> normal[NUM_XCD], lng[NUM_XCD];
+    nlong = 0;
+    for (uint32_t g = 0; g < ngroups; g++)
+        collect_blocks(gbeg[g], gbeg[g + 1], uptr, umax, rmax, long_units, long_max,
+                       normal[g % NUM_XCD], lng[g % NUM_XCD], nlong);
+    size_t rounds = 0, nmax = 0;
+    splits.clear();
+    npieces = 0;
+    for (int x = 0; x < NUM_XCD; x++) {
+        sort_long(lng[x]);
+        int st = number_pieces(lng[x], splits, npieces);
+        if (st) return st;
+        rounds = std::max(rounds, (lng[x].size() + WAVES - 1) / WAVES);
+        nmax = std::max(nmax, normal[x].size());
+    }
+    nmax = (nmax + WAVES - 1) / WAVES * WAVES;
+    blocks.clear();
+    for (size_t r = 0; r < rounds; r++)
+        for (int x = 0; x < NUM_XCD; x++)
+            for (int w = 0; w < WAVES; w++) {
+                const size_t i = r * WAVES + w;
+                blocks.push_back(i < lng[x].size() ? lng[x][i] : LONG_FILL);
+            }
+    for (int x = 0; x < NUM_XCD; x++) {
+        const uint32_t u = normal[x].empty() ? 0 : normal[x].back().uend;
+        const uint32_t rr = normal[x].empty() ? 0 : normal[x].back().rend;
+        blocks.insert(blocks.end(), normal[x].begin(), normal[x].end());
+        for (size_t i = normal[x].size(); i < nmax; i++) blocks.push_back({u, u, rr, rr, 0, 0, 0, 0});
+    }
+    nlong_wg = (uint32_t)(rounds * NUM_XCD);
     return INTSPMV_OK;
 }
 
@@ -360,17 +467,43 @@ extern "C" int intspmv_plan_layout(uint32_t nrows, const uint32_t *uptr, uint32_
     return INTSPMV_OK;
 }
 
+extern "C" int intspmv_plan_layout_grouped(uint32_t nrows, const uint32_t *uptr, uint32_t umax,
+                                           uint32_t rmax, uint32_t long_units, uint32_t long_max,
+                                           uint32_t ngroups, const uint32_t *group_rows,
+                                           uint32_t *blocks, size_t cap_blocks, uint32_t *nblocks,
+                                           uint32_t *splits, size_t cap_splits, uint32_t *nsplits,
+                                           uint32_t *nlong_wg)
+{
+    std::vector<ispv::BlockDesc> b;
+    std::vector<ispv::SplitDesc> sp;
+    uint32_t nlw = 0, nlong = 0, npieces = 0;
+    int st = ispv_build_blocks_grouped(nrows, uptr, umax, rmax, long_units, long_max, ngroups,
+                                       group_rows, b, sp, nlw, nlong, npieces);
+    if (st) return st;
+    if (nblocks) *nblocks = (uint32_t)b.size();
+    if (nsplits) *nsplits = (uint32_t)sp.size();
+    if (nlong_wg) *nlong_wg = nlw;
+    if (blocks && cap_blocks >= b.size())
+        memcpy(blocks, b.data(), b.size() * sizeof(ispv::BlockDesc));
+    if (splits && cap_splits >= sp.size())
+        memcpy(splits, sp.data(), sp.size() * sizeof(ispv::SplitDesc));
+    return INTSPMV_OK;
+}
+
 int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
                     uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_max,
-                    intspmv_plan **out)
+                    intspmv_plan **out, uint32_t ngroups, const uint32_t *gbeg)
 {
     if (!out) return INTSPMV_ERR_ARG;
     *out = nullptr;
     std::vector<ispv::BlockDesc> blocks;
     std::vector<ispv::SplitDesc> splits;
     uint32_t nlw = 0, nlong = 0, npieces = 0;
-    int st = ispv_build_blocks(nrows, uptr, umax, rmax, long_units, long_max, blocks, splits, nlw,
-                          nlong, npieces);
+    int st = ngroups ? ispv_build_blocks_grouped(nrows, uptr, umax, rmax, long_units, long_max,
+                                                 ngroups, gbeg, blocks, splits, nlw, nlong,
+                                                 npieces)
+                     : ispv_build_blocks(nrows, uptr, umax, rmax, long_units, long_max, blocks,
+                                         splits, nlw, nlong, npieces);
     if (st) return st;
     intspmv_plan *p = (intspmv_plan *)calloc(1, sizeof(intspmv_plan));
     if (!p) return INTSPMV_ERR_NOMEM;

@@ -1032,8 +1032,8 @@ static int launch_noc(const intspmv_plan *p, const void *ax, const void *xp, con
 
 using namespace ispv;
 
-extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
-                                const uint32_t *con_host, intspmv_plan **plan)
+static int axt_plan_impl(intspmv_ctx *ctx, const intspmv_axt_desc *desc, const uint32_t *con_host,
+                         uint32_t ngroups, const uint32_t *gbeg, intspmv_plan **plan)
 {
     if (!ctx || !desc || !plan || (desc->nrows && !con_host)) return INTSPMV_ERR_ARG;
     const intspmv_axt_desc &D = *desc;
@@ -1062,7 +1062,7 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
         }
         uptr[D.nrows] = D.nnz;
         int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, COM_UMAX, AXT_RMAX, COM_UMAX / 2,
-                                 LONG_SLOTS, plan);
+                                 LONG_SLOTS, plan, ngroups, gbeg);
         free(uptr);
         if (st) return st;
         (*plan)->axt = D;
@@ -1118,11 +1118,25 @@ extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
                             : D.layout == INTSPMV_AXT_COLMAP ? GATHER_H1_LONG_MAX
                                                              : LONG_SLOTS / 8;
     int st = ispv_plan_build(ctx, PLAN_AXT, D.nrows, uptr, umax, AXT_RMAX, long_units, long_max,
-                             plan);
+                             plan, ngroups, gbeg);
     free(uptr);
     if (st) return st;
     (*plan)->axt = D;
     return INTSPMV_OK;
+}
+
+extern "C" int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
+                                const uint32_t *con_host, intspmv_plan **plan)
+{
+    return axt_plan_impl(ctx, desc, con_host, 0, nullptr, plan);
+}
+
+extern "C" int intspmv_axt_plan_grouped(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
+                                        const uint32_t *con_host, uint32_t ngroups,
+                                        const uint32_t *group_rows, intspmv_plan **plan)
+{
+    if (ngroups == 0 || !desc || desc->layout != INTSPMV_AXT_COLMAP) return INTSPMV_ERR_ARG;
+    return axt_plan_impl(ctx, desc, con_host, ngroups, group_rows, plan);
 }
 
 extern "C" int spmv_axt(intspmv_ctx *ctx, const intspmv_plan *plan, const void *ax,

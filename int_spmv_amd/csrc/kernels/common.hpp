@@ -190,9 +190,11 @@ int ispv_plan_upload(intspmv_ctx *ctx, intspmv_plan *p, const void *blocks, size
                      size_t nsplit);
 int ispv_split_fixup(const intspmv_plan *p, void *y, int y_is_f32, uint32_t k = 1,
                      uint32_t ldy = 1, uint32_t j0 = 0);
+// ngroups > 0: rows in groups [gbeg[g], gbeg[g+1]) pinned to XCD g % 8
+// (ispv_build_blocks_grouped).
 int ispv_plan_build(intspmv_ctx *ctx, int kind, uint32_t nrows, const uint32_t *uptr,
                     uint32_t umax, uint32_t rmax, uint32_t long_units, uint32_t long_max,
-                    intspmv_plan **out);
+                    intspmv_plan **out, uint32_t ngroups = 0, const uint32_t *gbeg = nullptr);
 #include <vector>
 // host-only block builder (abi.hip), shared by the single-matrix and batch plans
 // True when more than a quarter of the rows own no unit (uptr[r] == uptr[r+1]).
@@ -208,6 +210,11 @@ int ispv_build_blocks(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint3
                       uint32_t long_units, uint32_t long_max, std::vector<ispv::BlockDesc> &blocks,
                       std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg, uint32_t &nlong,
                       uint32_t &npieces);
+int ispv_build_blocks_grouped(uint32_t nrows, const uint32_t *uptr, uint32_t umax, uint32_t rmax,
+                              uint32_t long_units, uint32_t long_max, uint32_t ngroups,
+                              const uint32_t *gbeg, std::vector<ispv::BlockDesc> &blocks,
+                              std::vector<ispv::SplitDesc> &splits, uint32_t &nlong_wg,
+                              uint32_t &nlong, uint32_t &npieces);
 // grid size of a block plan
 static inline unsigned ispv_grid(const intspmv_plan *p)
 {

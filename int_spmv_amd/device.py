@@ -72,6 +72,8 @@ SIGNATURES = [
     ("intspmv_k1_build_device", C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                           _u32, C.POINTER(_u32)]),
     ("intspmv_axt_plan", C.c_int, [_vp, C.POINTER(AxtDesc), _vp, C.POINTER(_vp)]),
+    ("intspmv_axt_plan_grouped", C.c_int,
+     [_vp, C.POINTER(AxtDesc), _vp, _u32, _vp, C.POINTER(_vp)]),
     ("spmv_axt", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_axt_gather", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32, _u32]),
     ("spmv_axt_gather_permuted", C.c_int,
@@ -93,6 +95,9 @@ SIGNATURES = [
     ("intspmv_plan_layout", C.c_int, [_u32, _vp, _u32, _u32, _u32, _u32, _vp, C.c_size_t,
                                       C.POINTER(_u32), _vp, C.c_size_t, C.POINTER(_u32),
                                       C.POINTER(_u32)]),
+    ("intspmv_plan_layout_grouped", C.c_int,
+     [_u32, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _vp, C.c_size_t, C.POINTER(_u32), _vp,
+      C.c_size_t, C.POINTER(_u32), C.POINTER(_u32)]),
 ]
 
 
@@ -111,6 +116,30 @@ def plan_layout(uptr: np.ndarray, umax: int, rmax: int, long_units: int, long_ma
                               nb.value, C.byref(nb), _np_ptr(s), ns.value, C.byref(ns),
                               C.byref(nl))
     return b[: nb.value], s[: ns.value], nl.value
+
+def plan_layout_grouped(uptr: np.ndarray, umax: int, rmax: int, long_units: int, long_max: int,
+                        group_rows: np.ndarray | None, ngroups: int = 8):
+    """Host-only: (blocks[nb, 8], splits[ns, 4], nlong_wg) of a plan whose row
+    groups run on XCD g % 8 (group_rows None: ngroups groups of about equal
+    units)."""
+    uptr = np.ascontiguousarray(uptr, dtype=np.uint32)
+    nb, ns, nl = _u32(), _u32(), _u32()
+    n = len(uptr) - 1
+    if group_rows is None:
+        args = (n, _np_ptr(uptr), umax, rmax, long_units, long_max, ngroups, None)
+    else:
+        g = np.ascontiguousarray(group_rows, dtype=np.uint32)
+        args = (n, _np_ptr(uptr), umax, rmax, long_units, long_max, len(g) - 1, _np_ptr(g))
+    st = lib().intspmv_plan_layout_grouped(*args, None, 0, C.byref(nb), None, 0, C.byref(ns),
+                                           C.byref(nl))
+    if st:
+        raise IntSpmvError(st, "plan_layout_grouped")
+    b = np.zeros((max(nb.value, 1), 8), np.uint32)
+    s = np.zeros((max(ns.value, 1), 4), np.uint32)
+    lib().intspmv_plan_layout_grouped(*args, _np_ptr(b), nb.value, C.byref(nb), _np_ptr(s),
+                                      ns.value, C.byref(ns), C.byref(nl))
+    return b[: nb.value], s[: ns.value], nl.value
+
 
 _lib = None
 
@@ -423,7 +452,8 @@ class DeviceAXTGather:
     """AXT NOC with a column map: values and columns in slot order, x separate
     (n x k row-major, 1 <= k <= 32).  ``a`` must be built with want_acol."""
 
-    def __init__(self, eng: Engine, a, val_type: int = F64, x_type: int = F64, vals=None):
+    def __init__(self, eng: Engine, a, val_type: int = F64, x_type: int = F64, vals=None,
+                 placement="plain"):
         if a.mode != "NOC":
             raise ValueError("the column-map kernels take NOC layouts")
         self.eng, self.nrows, self.a = eng, a.nrows, a
@@ -433,11 +463,67 @@ class DeviceAXTGather:
         self.aval = vals if vals is not None else eng.upload(_value_plane(a, val_type))
         self.acol = eng.upload(a.acol)
         self.sec = eng.upload(a.sec)
+        self.plans = {}
+        self.set_placement(placement)
+
+    def _make_plan(self, placement) -> _Plan:
         h = _vp()
-        con = np.ascontiguousarray(a.con)
-        eng._check(lib().intspmv_axt_plan(eng.h, C.byref(self.desc), _np_ptr(con), C.byref(h)),
-                   "axt_plan")
-        self.plan = _Plan(eng, h)
+        con = np.ascontiguousarray(self.a.con)
+        if isinstance(placement, str) and placement == "plain":
+            self.eng._check(lib().intspmv_axt_plan(self.eng.h, C.byref(self.desc), _np_ptr(con),
+                                                   C.byref(h)), "axt_plan")
+        elif isinstance(placement, str) and placement == "xcd":
+            # eight row groups of about equal tiles, group g on XCD g
+            self.eng._check(lib().intspmv_axt_plan_grouped(self.eng.h, C.byref(self.desc),
+                                                           _np_ptr(con), 8, None, C.byref(h)),
+                            "axt_plan_grouped")
+        elif isinstance(placement, str):
+            raise ValueError(f"unknown placement {placement!r}")
+        else:
+            # rows [placement[g], placement[g+1]) run on XCD g % 8
+            g = np.ascontiguousarray(placement, dtype=np.uint32)
+            self.eng._check(lib().intspmv_axt_plan_grouped(self.eng.h, C.byref(self.desc),
+                                                           _np_ptr(con), len(g) - 1, _np_ptr(g),
+                                                           C.byref(h)), "axt_plan_grouped")
+        return _Plan(self.eng, h)
+
+    def set_placement(self, placement) -> None:
+        """Which plan the products use: "plain" (intspmv_axt_plan: long rows
+        dealt round-robin over the XCDs, normal blocks in contiguous eighths),
+        "xcd" (intspmv_axt_plan_grouped: eight row groups of equal tiles, each
+        with its long rows on one XCD) or explicit group row starts.  Same
+        bits either way."""
+        key = placement if isinstance(placement, str) else "custom"
+        if key == "custom" or key not in self.plans:
+            self.plans[key] = self._make_plan(placement)
+        self.plan = self.plans[key]
+        self.placement = key
+
+    def tune_placement(self, k: int = 1, reps: int = 10) -> dict:
+        """Times the product with k right-hand sides under the "plain" and
+        "xcd" plans (random X, the engine's stream) and keeps the faster.
+        Returns both times in microseconds."""
+        torch = _torch()
+        dt = torch.float64 if self.x_type == F64 else torch.float32
+        n = self.nrows
+        X = torch.rand(max(n * k, 1), dtype=dt, device=self.eng.torch_device)
+        Y = torch.empty(max(n * k, 1), dtype=dt, device=self.eng.torch_device)
+        times = {}
+        for pl in ("plain", "xcd"):
+            self.set_placement(pl)
+            for _ in range(2):
+                self.matvec(X, Y, k)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(self.eng.stream)
+            for _ in range(reps):
+                self.matvec(X, Y, k)
+            e1.record(self.eng.stream)
+            e1.synchronize()
+            times[pl] = e0.elapsed_time(e1) * 1e3 / reps
+        self.set_placement(min(times, key=times.get))
+        return dict(placement=self.placement, plain_us=round(times["plain"], 2),
+                    xcd_us=round(times["xcd"], 2))
 
     def matvec(self, x, y, k: int = 1, ldx: int | None = None, ldy: int | None = None,
                vals=None, kc: int = 0) -> None:
