@@ -552,8 +552,14 @@ def main():
         y = nanv(n)
         w, e = timed(lambda: g.matvec(xd, y), 20, 3, torch)
         fwd_g = e
+        # the ceilings of a product that gathers x once per entry: random
+        # 8-byte gathers within one XCD's L2 and over all of x
+        ceil = {"l2_1MiB_G_per_s": round(eng.gather_rate(1 << 20, 8) / 1e9, 1),
+                "x_span_G_per_s": round(eng.gather_rate(n * 8, 8) / 1e9, 1),
+                "x_span_bytes": n * 8}
         record("AXT_GATHER_H1_F64_K1", M.bytes_axt_gather(n, ag.lenAX, ag.lenSEC), w, e,
-               {"vs_embedded": round(e / head_ev, 3), "placement": pl1})
+               {"vs_embedded": round(e / head_ev, 3), "placement": pl1,
+                "G_gathers_per_s": round(nnz / e / 1e9, 1), "gather_ceilings": ceil})
         keep("AXT_GATHER_H1_F64_K1", "y", y, n)
         # conversion: a new x into the embedded AXT arrays, on the device
         # (spmv_axt_refresh_x) versus the host builder (getFormatDataAXT)

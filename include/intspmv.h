@@ -278,6 +278,16 @@ int intspmv_batch_destroy(intspmv_batch *batch);
 int spmv_axt_h1_noalloc(intspmv_ctx *ctx, uint32_t nrows, uint32_t tileN, const double *ax,
                         const uint32_t *sec, double *y);
 
+/* Measurement utility, no reference counterpart: the rate (gathers per
+ * second) of a kernel that streams `count` 4-byte indexes and gathers one
+ * `elem_bytes`-wide element (4, 8 or 16) at each, uniformly over a buffer of
+ * `span_bytes`, timed over `reps` launches on the context's stream.  It is
+ * the ceiling of a product whose entries each gather x alone (the column
+ * map, CSR, K1): a span inside one XCD's L2 gives the L2 request ceiling,
+ * the span of x the ceiling for uniformly placed columns.  Allocates and
+ * frees its own buffers; synchronises the stream. */
+int intspmv_gather_rate(intspmv_ctx *ctx, size_t span_bytes, uint32_t elem_bytes, size_t count,
+                        uint32_t reps, double *gathers_per_s);
 int intspmv_plan_destroy(intspmv_plan *plan);
 /* Host-only inspection of the work decomposition a plan uses for a row ->
  * unit pointer uptr[nrows+1]: blocks (8 words each: ubeg, uend, rbeg, rend,

@@ -90,6 +90,8 @@ SIGNATURES = [
     ("intspmv_batch_info", C.c_int, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
     ("intspmv_batch_destroy", C.c_int, [_vp]),
     ("spmv_axt_h1_noalloc", C.c_int, [_vp, _u32, _u32, _vp, _vp, _vp]),
+    ("intspmv_gather_rate", C.c_int, [_vp, C.c_size_t, _u32, C.c_size_t, _u32,
+                                      C.POINTER(C.c_double)]),
     ("intspmv_plan_destroy", C.c_int, [_vp]),
     ("intspmv_plan_info", C.c_int, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
     ("intspmv_plan_layout", C.c_int, [_u32, _vp, _u32, _u32, _u32, _u32, _vp, C.c_size_t,
@@ -208,6 +210,15 @@ class Engine:
         cur = _torch().cuda.current_stream(self.torch_device)
         if cur.cuda_stream != self.stream.cuda_stream:
             self.set_stream(cur)
+
+    def gather_rate(self, span_bytes: int, elem_bytes: int = 8, count: int = 1 << 25,
+                    reps: int = 10) -> float:
+        """Random gathers per second over a buffer of span_bytes
+        (intspmv_gather_rate): the ceiling of gather-bound products."""
+        r = C.c_double()
+        self._check(lib().intspmv_gather_rate(self.h, span_bytes, elem_bytes, count, reps,
+                                              C.byref(r)), "gather_rate")
+        return r.value
 
     def sync(self) -> None:
         self._check(lib().intspmv_sync(self.h), "sync")

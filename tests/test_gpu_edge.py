@@ -142,3 +142,17 @@ def test_bad_arguments_are_refused_before_launch(engine):
         assert ei.value.status == -1, kw
     torch.cuda.synchronize()
     assert torch.isnan(Y).all()
+
+
+def test_gather_rate_calibration():
+    """intspmv_gather_rate (the gather ceiling bench.py reports) returns a
+    positive rate for every element size and refuses bad arguments without
+    launching."""
+    eng = D.Engine(0)
+    for elem in (4, 8, 16):
+        r = eng.gather_rate(1 << 20, elem, count=1 << 20, reps=2)
+        assert 1e9 < r < 1e14, (elem, r)
+    for span, elem, count, reps in ((1 << 20, 3, 1 << 10, 1), (4, 8, 1 << 10, 1),
+                                    (1 << 20, 8, 0, 1), (1 << 20, 8, 1 << 10, 0)):
+        with pytest.raises(D.IntSpmvError):
+            eng.gather_rate(span, elem, count=count, reps=reps)
