@@ -163,7 +163,12 @@ __device__ __forceinline__ void gather_block(const BlockDesc &d, const VT *__res
     LaneSum<AT> lacc[KC]; // long rows; FP32 sums compensated (Neumaier)
     // tiles of this block; UG steps of 16 tiles are loaded before their
     // gathers are issued, so a lane keeps 2*UG gathers in flight
-    constexpr int UG = KC >= 8 ? 1 : (KC >= 4 ? UG4 : (KC == 1 ? UG1 : 4));
+    // (th > 1: a block holds at most G_UMAX = 128 tile columns, i.e. 16
+    // tiles, 17 when it starts inside a tile, so two groups cover it; more
+    // groups would only reload the block's last tile, clamped)
+    constexpr int UGH = KC >= 8 ? 1 : (KC >= 4 ? UG4 : (KC == 1 ? UG1 : 4));
+    constexpr int UG = H1 ? UGH : (UGH < 2 ? UGH : 2);
+    static_assert(H1 || G_UMAX <= 16 * 8, "th > 1: a block spans at most 17 tiles");
     const uint32_t T0 = H1 ? d.ubeg : d.ubeg / 8, T1 = H1 ? d.uend : (d.uend + 7) / 8;
     for (uint32_t Tbb = T0; Tbb < T1; Tbb += 16 * UG) {
         AT a0[UG][KC], a1[UG][KC];
