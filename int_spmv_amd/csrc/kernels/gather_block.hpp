@@ -2,6 +2,9 @@
 // shared by the single-matrix kernel (gather.hip) and the batched persistent
 // kernel (batch.hip).
 #pragma once
+#include <type_traits>
+#include <utility>
+
 #include "common.hpp"
 #include "reduce.hpp"
 
@@ -434,6 +437,76 @@ __device__ __forceinline__ void tile_values(const VT *aval, size_t T, AT (&v)[8]
     }
 }
 
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+
+// ds_swizzle in bit mode over each 32-lane half: every lane reads lane
+// (lane & AND) | OR.  Groups of G lanes (G a power of two <= 32): AND keeps
+// the group's bits, OR names the lane within the group.
+template <int G, int SRC>
+__device__ __forceinline__ uint32_t group_bcast(uint32_t v)
+{
+    static_assert(G >= 2 && G <= 32 && (G & (G - 1)) == 0 && SRC < G, "lane within the group");
+    constexpr int AND = 0x1F & ~(G - 1);
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, AND | (SRC << 5));
+}
+template <int G, int SRC, typename AT>
+__device__ __forceinline__ AT group_bcast_val(AT v)
+{
+    if constexpr (sizeof(AT) == 8) {
+        const uint32_t lo = group_bcast<G, SRC>((uint32_t)__double2loint(v));
+        const uint32_t hi = group_bcast<G, SRC>((uint32_t)__double2hiint(v));
+        return __hiloint2double((int)hi, (int)lo);
+    } else {
+        return __uint_as_float(group_bcast<G, SRC>(__float_as_uint(v)));
+    }
+}
+
+template <int G, int P, typename AT, int... Q>
+__device__ __forceinline__ void bcast_slots(std::integer_sequence<int, Q...>, const AT (&vo)[P],
+                                            const uint32_t (&co)[P], AT (&v)[8],
+                                            uint32_t (&c)[8])
+{
+    ((c[Q] = group_bcast<G, Q / P>(co[Q % P]), v[Q] = group_bcast_val<G, Q / P, AT>(vo[Q % P])),
+     ...);
+}
+
+// The eight values and columns of tile T in every lane of a group of G lanes,
+// each slot read from memory by one lane only: lane i of the group loads
+// slots P i .. P i + P - 1 (P = 8 / G; for G > 8, lanes i and i + 8 the same
+// slot) and the group exchanges them by lane swizzles.  The same operands as
+// tile_values and a load of all eight columns per lane, with an eighth of
+// the loads' lane traffic.
+template <typename VT, typename AT, int G>
+__device__ __forceinline__ void tile_slots_group(const VT *__restrict__ aval,
+                                                 const uint32_t *__restrict__ acol, size_t T,
+                                                 uint32_t li, AT (&v)[8], uint32_t (&c)[8])
+{
+    constexpr int P = G >= 8 ? 1 : 8 / G;
+    const size_t s0 = T * 8 + (G >= 8 ? (li & 7u) : li * P);
+    AT vo[P];
+    uint32_t co[P];
+    if constexpr (P == 1) {
+        co[0] = ntload(acol + s0);
+        if constexpr (std::is_same<VT, uint16_t>::value) vo[0] = bf16_to_f32(ntload(aval + s0));
+        else vo[0] = (AT)ntload(aval + s0);
+    } else {
+        typedef uint32_t uPv __attribute__((ext_vector_type(P)));
+        const uPv cc = ntload(reinterpret_cast<const uPv *>(acol + s0));
+#pragma unroll
+        for (int p = 0; p < P; p++) co[p] = cc[p];
+        if constexpr (std::is_same<VT, uint16_t>::value) {
+#pragma unroll
+            for (int p = 0; p < P; p++) vo[p] = bf16_to_f32(ntload(aval + s0 + p));
+        } else {
+            typedef VT vPv __attribute__((ext_vector_type(P)));
+            const vPv vv = ntload(reinterpret_cast<const vPv *>(aval + s0));
+#pragma unroll
+            for (int p = 0; p < P; p++) vo[p] = (AT)vv[p];
+        }
+    }
+    bcast_slots<(G > 8 ? 8 : G), P>(std::make_integer_sequence<int, 8>{}, vo, co, v, c);
+}
+
 // V values of x row c from right-hand side j (0 for padding and for the
 // values at or past cnt = right-hand sides left for this lane), without
 // branches, so that the eight gathers of a tile stay in flight together.
@@ -493,16 +566,11 @@ __device__ __forceinline__ void gather_block_wide(const BlockDesc &d, const VT *
             const uint32_t t = b0 + grp + NG * u;
             const size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
             AT v[8];
-            tile_values<VT, AT>(aval, T, v);
-            typedef uint32_t u4v __attribute__((ext_vector_type(4)));
-            const u4v c0 = ntload(reinterpret_cast<const u4v *>(acol + T * 8));
-            const u4v c1 = ntload(reinterpret_cast<const u4v *>(acol + T * 8 + 4));
+            uint32_t c[8];
+            tile_slots_group<VT, AT, G>(aval, acol, T, lane % G, v, c);
             AT xv[8][V];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                wide_x<XT, AT, V, XV>(x, c0[q], ldx, j0 + jl, cnt, xv[q]);
-                wide_x<XT, AT, V, XV>(x, c1[q], ldx, j0 + jl, cnt, xv[q + 4]);
-            }
+            for (int q = 0; q < 8; q++) wide_x<XT, AT, V, XV>(x, c[q], ldx, j0 + jl, cnt, xv[q]);
 #pragma unroll
             for (int e = 0; e < V; e++) {
                 AT pr[8];
@@ -587,7 +655,6 @@ __device__ __forceinline__ void gather_long_wide(const BlockDesc &d, const VT *_
             const size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
             AT v[8];
             tile_values<VT, AT>(aval, T, v);
-            typedef uint32_t u4v __attribute__((ext_vector_type(4)));
             const u4v c0 = ntload(reinterpret_cast<const u4v *>(acol + T * 8));
             const u4v c1 = ntload(reinterpret_cast<const u4v *>(acol + T * 8 + 4));
             AT xv[8][V];
