@@ -664,8 +664,10 @@ def main():
         del g32, adj32, ag, x32, y32, yb32, xb32
         dc = D.DeviceCSR(eng, m)
         y = nanv(n)
+        plc = dc.tune_placement(xd, y)
+        y.fill_(float("nan"))
         w, e = timed(lambda: dc.spmv(xd, y), 20, 3, torch)
-        record("CSR_F64", M.bytes_csr(n, nnz), w, e)
+        record("CSR_F64", M.bytes_csr(n, nnz), w, e, {"placement": plc})
         keep("CSR_F64", "y", y, n)
         y.fill_(float("nan"))
         w, e = timed(lambda: dc.cf_csr(xd, y), 3, 1, torch)

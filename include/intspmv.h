@@ -72,6 +72,11 @@ int intspmv_memcpy_dtoh(intspmv_ctx *ctx, void *dst, const void *src, size_t byt
  *   are reduced by a wave tree and split across waves. */
 int intspmv_csr_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row_host,
                      intspmv_plan **plan);
+/* The same plan with rows in groups pinned to XCDs, as intspmv_axt_plan_grouped
+ * (group g on XCD g % 8; group_rows = NULL: ngroups groups of equal entries).
+ * spmv_csr gives the same bits with either plan. */
+int intspmv_csr_plan_grouped(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row_host,
+                             uint32_t ngroups, const uint32_t *group_rows, intspmv_plan **plan);
 int spmv_csr_ordered(intspmv_ctx *ctx, uint32_t nrows, const double *val, const uint32_t *col,
                      const uint32_t *row, const double *x, double *y);
 int spmv_csr(intspmv_ctx *ctx, const intspmv_plan *plan, const double *val,

@@ -175,6 +175,15 @@ extern "C" int intspmv_csr_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t
                            CSR_LONG_MAX, plan);
 }
 
+extern "C" int intspmv_csr_plan_grouped(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row_host,
+                                        uint32_t ngroups, const uint32_t *group_rows,
+                                        intspmv_plan **plan)
+{
+    if (!ctx || !plan || !row_host || row_host[0] != 0 || ngroups == 0) return INTSPMV_ERR_ARG;
+    return ispv_plan_build(ctx, PLAN_CSR, nrows, row_host, CSR_UMAX, CSR_RMAX, CSR_LONG,
+                           CSR_LONG_MAX, plan, ngroups, group_rows);
+}
+
 extern "C" int spmv_csr_ordered(intspmv_ctx *ctx, uint32_t nrows, const double *val,
                                 const uint32_t *col, const uint32_t *row, const double *x,
                                 double *y)

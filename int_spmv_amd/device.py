@@ -62,6 +62,7 @@ SIGNATURES = [
     ("intspmv_memcpy_htod", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     ("intspmv_memcpy_dtoh", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     ("intspmv_csr_plan", C.c_int, [_vp, _u32, _vp, C.POINTER(_vp)]),
+    ("intspmv_csr_plan_grouped", C.c_int, [_vp, _u32, _vp, _u32, _vp, C.POINTER(_vp)]),
     ("spmv_csr_ordered", C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_csr", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("intspmv_axc_plan", C.c_int, [_vp, _u32, _vp, C.POINTER(_vp)]),
@@ -264,15 +265,53 @@ class _Plan:
 
 
 class DeviceCSR:
-    def __init__(self, eng: Engine, m):
+    def __init__(self, eng: Engine, m, placement: str = "plain"):
         self.eng, self.nrows, self.nnz = eng, m.nrows, m.nnz
         self.val = eng.upload(m.val)
         self.col = eng.upload(m.col)
         self.row = eng.upload(m.row)
-        h = _vp()
-        row = np.ascontiguousarray(m.row)
-        eng._check(lib().intspmv_csr_plan(eng.h, m.nrows, _np_ptr(row), C.byref(h)), "csr_plan")
-        self.plan = _Plan(eng, h)
+        self._row_host = np.ascontiguousarray(m.row)
+        self.plans = {}
+        self.set_placement(placement)
+
+    def set_placement(self, placement: str) -> None:
+        """"plain" (intspmv_csr_plan) or "xcd" (intspmv_csr_plan_grouped, eight
+        row groups of equal entries, each on one XCD); same bits."""
+        if placement not in self.plans:
+            h = _vp()
+            row = self._row_host
+            if placement == "plain":
+                self.eng._check(lib().intspmv_csr_plan(self.eng.h, self.nrows, _np_ptr(row),
+                                                       C.byref(h)), "csr_plan")
+            elif placement == "xcd":
+                self.eng._check(lib().intspmv_csr_plan_grouped(self.eng.h, self.nrows,
+                                                               _np_ptr(row), 8, None, C.byref(h)),
+                                "csr_plan_grouped")
+            else:
+                raise ValueError(f"unknown placement {placement!r}")
+            self.plans[placement] = _Plan(self.eng, h)
+        self.plan = self.plans[placement]
+        self.placement = placement
+
+    def tune_placement(self, x, y, reps: int = 10) -> dict:
+        """Times spmv under both plans and keeps the faster (microseconds)."""
+        torch = _torch()
+        times = {}
+        for pl in ("plain", "xcd"):
+            self.set_placement(pl)
+            for _ in range(2):
+                self.spmv(x, y)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(self.eng.stream)
+            for _ in range(reps):
+                self.spmv(x, y)
+            e1.record(self.eng.stream)
+            e1.synchronize()
+            times[pl] = e0.elapsed_time(e1) * 1e3 / reps
+        self.set_placement(min(times, key=times.get))
+        return dict(placement=self.placement, plain_us=round(times["plain"], 2),
+                    xcd_us=round(times["xcd"], 2))
 
     def cf_csr(self, x, y) -> None:
         """Ordered product, same bits as cf_CSR (intSpmv_main.c:382-396)."""

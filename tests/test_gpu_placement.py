@@ -109,3 +109,24 @@ def test_tune_placement_keeps_a_plan(eng):
     r = g.tune_placement(k=1, reps=3)
     assert r["placement"] in ("plain", "xcd") and g.placement == r["placement"]
     assert r["plain_us"] > 0 and r["xcd_us"] > 0
+
+
+@pytest.mark.parametrize("name,m", _matrices())
+def test_csr_grouped_plan_same_bits(eng, name, m):
+    """spmv_csr under intspmv_csr_plan_grouped equals the plain plan bit for
+    bit, and both are within 1e-12 of the oracle."""
+    n = m.nrows
+    x = H.uniform(n, -1.0, 1.0, seed=8)
+    dc = D.DeviceCSR(eng, m)
+    xd = eng.upload(x)
+    out = {}
+    for pl in ("plain", "xcd"):
+        dc.set_placement(pl)
+        y = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+        dc.spmv(xd, y)
+        torch.cuda.synchronize()
+        out[pl] = y.cpu()
+    assert torch.equal(out["plain"], out["xcd"]), name
+    yref = O.cf_csr(m.row, m.col, m.val, x)
+    err, row = O.scaled_err(m.row, m.col, m.val, x, yref, out["xcd"].numpy())
+    assert err <= 1e-12, f"{name}: {err} at row {row}"
