@@ -673,6 +673,12 @@ def main():
         w, e = timed(lambda: dc.cf_csr(xd, y), 3, 1, torch)
         record("CSR_ORDERED_F64", M.bytes_csr(n, nnz), w, e)
         keep("CSR_ORDERED_F64", "y", y, n, bitwise=True)
+        # cf_CSR's bits from the block kernel: an exact plan (no split rows,
+        # one fma chain per long row through LDS)
+        y.fill_(float("nan"))
+        w, e = timed(lambda: dc.spmv_exact(xd, y), 20, 3, torch)
+        record("CSR_EXACT_F64", M.bytes_csr(n, nnz), w, e)
+        keep("CSR_EXACT_F64", "y", y, n, bitwise=True)
         rs = rocsparse_leg(dc, xd, y, n, nnz)
         if rs is None:
             formats["ROCSPARSE_CSR_F64"] = "librocsparse not found"

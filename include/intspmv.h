@@ -72,6 +72,13 @@ int intspmv_memcpy_dtoh(intspmv_ctx *ctx, void *dst, const void *src, size_t byt
  *   are reduced by a wave tree and split across waves. */
 int intspmv_csr_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row_host,
                      intspmv_plan **plan);
+/* A plan under which spmv_csr gives the bits of cf_CSR (intSpmv_main.c:382-396)
+ * for every row: rows longer than a block are never split, and each is one fma
+ * chain in CSR order from +0.0, staged through LDS by one wave.  Rows of at
+ * least 2^31 entries are refused.  The plan-free spmv_csr_ordered gives the
+ * same bits with one lane per row. */
+int intspmv_csr_plan_exact(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row_host,
+                           intspmv_plan **plan);
 /* The same plan with rows in groups pinned to XCDs, as intspmv_axt_plan_grouped
  * (group g on XCD g % 8; group_rows = NULL: ngroups groups of equal entries).
  * spmv_csr gives the same bits with either plan. */

@@ -274,7 +274,7 @@ static void collect_blocks(uint32_t r0, uint32_t r1, const uint32_t *uptr, uint3
     while (r < r1) {
         uint32_t len = uptr[r + 1] - uptr[r];
         if (len > long_units) {
-            uint32_t np = (len + long_max - 1) / long_max;
+            uint32_t np = (uint32_t)(((uint64_t)len + long_max - 1) / long_max);
             if (np == 1) {
                 lng.push_back({uptr[r], uptr[r + 1], r, r + 1, 0, 1, 0, 0});
                 nlong++;

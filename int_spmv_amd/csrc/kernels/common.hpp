@@ -179,6 +179,7 @@ struct intspmv_plan {
     uint32_t npieces;
     ispv::SplitDesc *d_splits;  // [nsplit_rows]
     uint32_t chunkNum;  // K1
+    uint32_t exact;     // CSR: long rows unsplit, each one fma chain in CSR order
     intspmv_axt_desc axt;
 };
 

@@ -9,7 +9,9 @@
 //   from LDS -- so every row of a normal block has exactly the ordered bits.
 //   Rows longer than 256 entries are streamed by one wave each (split over
 //   several waves beyond 8192) and reduced by a wave tree, within the FP64
-//   tolerance.
+//   tolerance.  With an exact plan (intspmv_csr_plan_exact) long rows are not
+//   split: the wave stages 512 entries at a time in LDS and one lane runs the
+//   row's fma chain over them in CSR order, so every row has cf_CSR's bits.
 #include "common.hpp"
 #include "reduce.hpp"
 
@@ -42,7 +44,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_ordered_kernel(
 // A pair never reaches past the arrays: pair starts are clamped to the last
 // whole pair, and with odd nnz the last entry is fetched on its own (one
 // uniform load) and substituted.
-template <bool PAIRS>
+template <bool PAIRS, bool EXACT>
 __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
     const BlockDesc *__restrict__ blocks, uint32_t nlong_wg, uint32_t nblocks,
     const double *__restrict__ val, const uint32_t *__restrict__ col,
@@ -57,6 +59,34 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
     const BlockDesc d = blocks[bid];
     const uint32_t nu = d.uend - d.ubeg;
     constexpr int U = 8;
+    if (EXACT && d.k != 0) {
+        // long row, never split: chunks of CSR_UMAX entries staged in LDS
+        // (value, gathered x), then lane 0 continues the row's chain over them
+        double acc = 0.0;
+        for (uint32_t c0 = 0; c0 < nu; c0 += CSR_UMAX) {
+            const uint32_t m = nu - c0 < CSR_UMAX ? nu - c0 : CSR_UMAX;
+#pragma unroll
+            for (int j = 0; j < (int)(CSR_UMAX / WAVE); j++) {
+                uint32_t i = j * WAVE + lane;
+                const bool ok = i < m;
+                i = ok ? i : m - 1; // branch-free: re-read the chunk's last entry
+                const double v = ntload(val + d.ubeg + c0 + i);
+                const uint32_t c = ntload(col + d.ubeg + c0 + i);
+                const double xv = x[c];
+                if (ok) {
+                    s_v[w][i] = v;
+                    s_x[w][i] = xv;
+                }
+            }
+            wave_lds_sync();
+            if (lane == 0)
+                in_order_chunks<16, d2v>(0, m, [&](uint32_t i) { return d2v{s_v[w][i], s_x[w][i]}; },
+                                         [&](d2v e) { acc = fma(e[0], e[1], acc); });
+            wave_lds_sync(); // the next chunk overwrites the staging
+        }
+        if (lane == 0 && d.rbeg < nrows) y[d.rbeg] = acc;
+        return;
+    }
     if (d.k != 0) {
         // long row: lane-strided fma chains in registers, one wave tree
         double acc = 0.0;
@@ -175,6 +205,21 @@ extern "C" int intspmv_csr_plan(intspmv_ctx *ctx, uint32_t nrows, const uint32_t
                            CSR_LONG_MAX, plan);
 }
 
+extern "C" int intspmv_csr_plan_exact(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row_host,
+                                      intspmv_plan **plan)
+{
+    if (!ctx || !plan || !row_host || row_host[0] != 0) return INTSPMV_ERR_ARG;
+    // long rows in one piece each: rows of up to 2^31 entries
+    for (uint32_t r = 0; r < nrows; r++)
+        if (row_host[r + 1] < row_host[r] || row_host[r + 1] - row_host[r] >= 0x80000000u)
+            return INTSPMV_ERR_ARG;
+    int st = ispv_plan_build(ctx, PLAN_CSR, nrows, row_host, CSR_UMAX, CSR_RMAX, CSR_LONG,
+                             0x80000000u, plan);
+    if (st) return st;
+    (*plan)->exact = 1;
+    return INTSPMV_OK;
+}
+
 extern "C" int intspmv_csr_plan_grouped(intspmv_ctx *ctx, uint32_t nrows, const uint32_t *row_host,
                                         uint32_t ngroups, const uint32_t *group_rows,
                                         intspmv_plan **plan)
@@ -206,14 +251,18 @@ extern "C" int spmv_csr(intspmv_ctx *ctx, const intspmv_plan *plan, const double
     if (plan->nblocks == 0) return INTSPMV_OK;
     if (!row || !x || !y || (plan->nunits && (!val || !col))) return INTSPMV_ERR_ARG;
     const bool pairs = plan->nunits >= 2 && (uintptr_t)val % 16 == 0 && (uintptr_t)col % 8 == 0;
-    if (pairs)
-        csr_block_kernel<true><<<ispv_grid(plan), BLOCK_THREADS, 0, ctx->stream>>>(
-            plan->d_blocks, plan->nlong_wg, plan->nblocks, val, col, row, x, y, plan->nrows,
-            plan->d_partial, plan->nunits);
-    else
-        csr_block_kernel<false><<<ispv_grid(plan), BLOCK_THREADS, 0, ctx->stream>>>(
-            plan->d_blocks, plan->nlong_wg, plan->nblocks, val, col, row, x, y, plan->nrows,
-            plan->d_partial, plan->nunits);
+#define ISPV_CSR(P, E)                                                                          \
+    csr_block_kernel<P, E><<<ispv_grid(plan), BLOCK_THREADS, 0, ctx->stream>>>(                  \
+        plan->d_blocks, plan->nlong_wg, plan->nblocks, val, col, row, x, y, plan->nrows,        \
+        plan->d_partial, plan->nunits)
+    if (plan->exact) {
+        if (pairs) ISPV_CSR(true, true);
+        else ISPV_CSR(false, true);
+    } else {
+        if (pairs) ISPV_CSR(true, false);
+        else ISPV_CSR(false, false);
+    }
+#undef ISPV_CSR
     st = ispv_launch_check(ctx);
     if (st) return st;
     return ispv_split_fixup(plan, y, 0);

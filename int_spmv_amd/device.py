@@ -63,6 +63,7 @@ SIGNATURES = [
     ("intspmv_memcpy_dtoh", C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     ("intspmv_csr_plan", C.c_int, [_vp, _u32, _vp, C.POINTER(_vp)]),
     ("intspmv_csr_plan_grouped", C.c_int, [_vp, _u32, _vp, _u32, _vp, C.POINTER(_vp)]),
+    ("intspmv_csr_plan_exact", C.c_int, [_vp, _u32, _vp, C.POINTER(_vp)]),
     ("spmv_csr_ordered", C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_csr", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("intspmv_axc_plan", C.c_int, [_vp, _u32, _vp, C.POINTER(_vp)]),
@@ -322,6 +323,19 @@ class DeviceCSR:
     def spmv(self, x, y) -> None:
         self.eng._check(lib().spmv_csr(self.eng.h, self.plan.h, _dptr(self.val), _dptr(self.col),
                                        _dptr(self.row), _dptr(x), _dptr(y)), "spmv_csr")
+
+    def spmv_exact(self, x, y) -> None:
+        """spmv_csr under an exact plan (intspmv_csr_plan_exact): the bits of
+        cf_CSR (intSpmv_main.c:382-396) for every row, long rows included."""
+        if getattr(self, "_exact_plan", None) is None:
+            h = _vp()
+            self.eng._check(lib().intspmv_csr_plan_exact(self.eng.h, self.nrows,
+                                                         _np_ptr(self._row_host), C.byref(h)),
+                            "csr_plan_exact")
+            self._exact_plan = _Plan(self.eng, h)
+        self.eng._check(lib().spmv_csr(self.eng.h, self._exact_plan.h, _dptr(self.val),
+                                       _dptr(self.col), _dptr(self.row), _dptr(x), _dptr(y)),
+                        "spmv_csr (exact plan)")
 
 
 class DeviceAXC:

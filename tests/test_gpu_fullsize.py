@@ -101,6 +101,9 @@ def test_powerlaw_csr(engine, wl):
     short = np.diff(m.row.astype(np.int64)) <= 256
     assert bitwise_equal(g[short], yr[short])
     assert _serr(m, x, yr, g) <= TOL64
+    y3 = _nan(engine, m.nrows)
+    d.spmv_exact(xd, y3)  # exact plan: cf_CSR's bits for all rows
+    assert bitwise_equal(_host(y3, m.nrows), yr)
 
 
 def test_powerlaw_axc_k1(engine, wl):
@@ -289,6 +292,9 @@ def test_precision_workloads(engine, wl, name):
     y = _nan(engine, n)
     d.spmv(xd, y)
     errs["CSR_F64"] = _serr(m, x, yr, _host(y, n))
+    y = _nan(engine, n)
+    d.spmv_exact(xd, y)
+    assert bitwise_equal(_host(y, n), yr), f"{name}: exact CSR plan"
     del d
     c = H.AXC(m, x)
     dc = D.DeviceAXC(engine, c)

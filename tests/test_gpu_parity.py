@@ -70,6 +70,16 @@ def test_csr_block_lap256(engine, lap256):
     assert bitwise_equal(_host(y, m.nrows), O.cf_csr(m.row, m.col, m.val, x))
 
 
+def test_csr_exact_plan_bitwise(engine, mats):
+    """spmv_csr under intspmv_csr_plan_exact: cf_CSR's bits for every row,
+    rows of 9,000-30,000 entries (split under the plain plan) included."""
+    for name, m, x, yr in mats:
+        d = D.DeviceCSR(engine, m)
+        y = _y(engine, m.nrows)
+        d.spmv_exact(engine.upload(x), y)
+        assert bitwise_equal(_host(y, m.nrows), yr), name
+
+
 def test_csr_mats(engine, mats):
     for name, m, x, yr in mats:
         d = D.DeviceCSR(engine, m)

@@ -72,6 +72,9 @@ def test_random_matrix_all_products(engine, seed):
     d.spmv(xd, y2)
     assert bitwise_equal(_host(y1, n), yr), seed
     assert scaled(m, x, yr, _host(y2, n)) <= 1e-12, seed
+    y3 = _y(engine, n)
+    d.spmv_exact(xd, y3)  # the exact plan: cf_CSR's bits, long rows included
+    assert bitwise_equal(_host(y3, n), yr), seed
     # AXC
     a = H.AXC(m, x)
     y = _y(engine, n)
