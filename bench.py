@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--placement", choices=("tuned", "plain", "xcd"), default="tuned",
                    help="--gather-only: the column-map plan's placement (tuned: the faster)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-precision", action="store_true",
+                   help="skip the stencil / R-MAT precision runs (FP64, FP32, BF16 storage)")
     p.add_argument("--batch-count", type=int, default=256,
                    help="matrices in the batched-launch measurement (0 = skip)")
     p.add_argument("--batch-tune", choices=("device", "model"), default="model",
@@ -186,6 +188,71 @@ def bench_batch(eng, count, torch, log, tune="model"):
     log(f"[bench] BATCH: {r}")
     r["_check"] = [(m, e["X"], e["Y"], e["k"]) for (m, _), e in zip(mats, ents)]
     return r
+
+
+# The precision workloads: a 27-point stencil and an R-MAT graph, values
+# stored in FP64, FP32 or BF16 (x FP32 beside the last two).
+PRECISION_WORKLOADS = ("stencil27_128", "rmat_22")
+
+
+def bench_precision(eng, torch, log, names=PRECISION_WORKLOADS):
+    """The embedded AXT product at tile height 1 on each workload with values
+    stored in FP64 (the drop-in), FP32 (x FP32) and BF16 (x FP32), FP32 sums
+    (compensated on long rows) for the last two.  Returns the timings and the
+    outputs to check against the FP64 oracle of the unrounded matrix."""
+    import numpy as np
+
+    from int_spmv_amd import device as D
+    from int_spmv_amd import host as H
+    from int_spmv_amd import metrics as M
+    from int_spmv_amd.workloads import WORKLOADS
+    out, pending = {}, []
+    for name in names:
+        t0 = time.perf_counter()
+        m, x = WORKLOADS[name]()
+        a = H.AXT(m, x, th=1)
+        n = m.nrows
+        fb = M.bytes_axt(n, a.lenAX, a.lenSEC)
+        r, ys, fe = {}, {}, None
+        for vt, xt, prec, sv, sx, sy, dt, tol in (
+                (D.F64, D.F64, "F64", 8, 8, 8, torch.float64, 1e-12),
+                (D.F32, D.F32, "F32", 4, 4, 4, torch.float32, 1e-6),
+                (D.BF16, D.F32, "BF16", 2, 4, 4, torch.float32, 1e-2)):
+            d = D.DeviceAXT(eng, a, val_type=vt, x_type=xt)
+            y = torch.full((max(n, 1),), float("nan"), dtype=dt, device=eng.torch_device)
+            w, e = timed(lambda: d.int_axt(y), 20, 3, torch)
+            fe = e if prec == "F64" else fe
+            b = M.bytes_axt(n, a.lenAX, a.lenSEC, sv=sv, sx=sx, sy=sy)
+            r[prec] = {"us": round(e * 1e6, 2), "GBs": round(b / e / 1e9, 1),
+                       "pct_hbm": round(100 * b / e / 1e9 / M.HBM_PEAK_GBS, 1),
+                       "GBs_eq_fp64": round(fb / e / 1e9, 1), "vs_fp64": round(fe / e, 2),
+                       "tol": tol}
+            torch.cuda.synchronize()
+            ys[prec] = y[:n].double().cpu().numpy()
+            del d, y
+        out[name] = {"n": n, "nnz": m.nnz, "tileHeight": 1, "x": "FP64 for F64, FP32 else",
+                     "formats": r, "setup_s": round(time.perf_counter() - t0, 1)}
+        log(f"[bench] PRECISION {name}: {r}")
+        pending.append((name, m, np.asarray(x), ys))
+        del a
+    return out, pending
+
+
+def check_precision(out, pending):
+    """Max per-row scaled error of every precision result against the FP64
+    oracle (cf_CSR) of the unrounded matrix; returns the names over their
+    tolerance."""
+    import oracle as O
+    over = []
+    for name, m, x, ys in pending:
+        yref = O.cf_csr(m.row, m.col, m.val, x)
+        for prec, y in ys.items():
+            e = O.scaled_err(m.row, m.col, m.val, x, yref, y)[0]
+            f = out[name]["formats"][prec]
+            f["max_scaled_err_vs_fp64_oracle"] = e
+            if not e <= f["tol"]:
+                over.append(f"{name}:{prec}")
+    return over
 
 
 def check_batch(items):
@@ -719,6 +786,9 @@ def main():
     batch = None
     if not args.quick and args.batch_count > 0:
         batch = bench_batch(eng, args.batch_count, torch, log, args.batch_tune)
+    precision, prec_pending = None, []
+    if not args.quick and not args.no_precision:
+        precision, prec_pending = bench_precision(eng, torch, log)
 
     # ---- CPU baseline + correctness (oracle as the checker) -------------------
     cpu, over = None, None
@@ -729,6 +799,8 @@ def main():
             batch.update(check_batch(batch.pop("_check")))
             if not batch["max_scaled_err_vs_fp32_oracle"] <= 1e-6:
                 over.append("batch")
+        if precision is not None:
+            over += check_precision(precision, prec_pending)
         if over:
             log(f"[bench] OVER TOLERANCE: {over}")
         cpu = cpu_leg(m, x, a1, hosts, head_bytes, args.workload, args.quick)
@@ -771,11 +843,14 @@ def main():
                      "cold_frac": (round(head_bytes / (cold_us * 1e-6) / 1e9 / M.HBM_PEAK_GBS, 4)
                                    if cold_us else None),
                      "kernel": "axt_h1_kernel<double,double,double,false> (+ split fix-up)"},
-        "errors": ({"checked": len(formats_checked(formats)), "over_tolerance": over}
+        "errors": ({"checked": len(formats_checked(formats)) + sum(
+                        "max_scaled_err_vs_fp64_oracle" in f for w in (precision or {}).values()
+                        for f in w["formats"].values()), "over_tolerance": over}
                    if over is not None else None),
         "cpu_baseline": cpu,
         "formats": formats,
         "batch": batch,
+        "precision": precision,
     }
     print(json.dumps(out), flush=True)
 
