@@ -238,6 +238,89 @@ def bench_precision(eng, torch, log, names=PRECISION_WORKLOADS):
     return out, pending
 
 
+def bench_differentiation(eng, m, torch, log):
+    """L(v, x) = y_bar^T A(v) x on the benchmark matrix through the autograd
+    operator (column map, tile height 1), FP64 and FP32: the largest relative
+    gap between central differences along 64 random directions (h = 1e-6 /
+    1e-3) and the directional derivative from the backward's (v_bar, x_bar);
+    and the times of the forward, of x_bar = A^T y_bar and of the whole
+    backward (x_bar reading the current values through the slot map, plus
+    v_bar), each against the forward.  L is bilinear, so the differences are
+    exact up to rounding; they are taken along the perturbations the inputs
+    actually carry after rounding to the storage type.  The times are also
+    given for the pair operator, which keeps A's values in A^T's slot order
+    too (both copies updated alike), so x_bar reads its own copy: x_bar
+    alone, and x_bar plus v_bar in both orders (its backward).  Each plan's
+    placement is tuned first."""
+    from int_spmv_amd import autograd as AG
+    from int_spmv_amd import device as D
+    out = {}
+    n = m.nrows
+    for vt, h, tol, name in ((D.F64, 1e-6, 1e-6, "F64"), (D.F32, 1e-3, 1e-4, "F32")):
+        op = AG.AxtOperator(eng, m, th=1, val_type=vt)
+        placement = op.tune_placement()
+        dev, dt = eng.torch_device, op.dtype
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(21)
+        v = op.initial_values().requires_grad_(True)
+        x = (torch.rand(n, device=dev, dtype=torch.float64, generator=gen) * 2 - 1).to(dt)
+        x.requires_grad_(True)
+        ybar = torch.randn(n, device=dev, dtype=torch.float64, generator=gen).to(dt)
+        (AG.spmv(v, x, op) * ybar).sum().backward()
+        gv, gx = v.grad.detach().double(), x.grad.detach().double()
+        occupied = (op.fwd.acol.view(torch.int32) != -1).to(torch.float64)
+        worst = 0.0
+        with torch.no_grad():
+            yb64 = ybar.double()
+            for _ in range(64):
+                dv = torch.randn(len(occupied), device=dev, dtype=torch.float64,
+                                 generator=gen) * occupied
+                dx = torch.randn(n, device=dev, dtype=torch.float64, generator=gen)
+                vp, vm = (v.double() + h * dv).to(dt), (v.double() - h * dv).to(dt)
+                xp, xm = (x.double() + h * dx).to(dt), (x.double() - h * dx).to(dt)
+                dv_e = (vp.double() - vm.double()) / (2 * h)
+                dx_e = (xp.double() - xm.double()) / (2 * h)
+                lp = (AG.spmv(vp, xp, op).double() * yb64).sum()
+                lm = (AG.spmv(vm, xm, op).double() * yb64).sum()
+                fd = ((lp - lm) / (2 * h)).item()
+                an = ((gv * dv_e).sum() + (gx * dx_e).sum()).item()
+                scale = ((gv * dv_e).abs().sum() + (gx * dx_e).abs().sum()).item()
+                worst = max(worst, abs(fd - an) / scale)
+            vd, xd = v.detach(), x.detach()
+            y = torch.empty(n, dtype=dt, device=dev)
+            gxb, gvb = torch.empty_like(xd), torch.empty_like(vd)
+            _, fwd = timed(lambda: op.fwd.matvec(xd, y, vals=vd), 20, 3, torch)
+            _, adj = timed(lambda: op.adj.adjoint(ybar, gxb, vals=vd), 20, 3, torch)
+
+            def backward():
+                op.adj.adjoint(ybar, gxb, vals=vd)
+                op.fwd.grad_vals(xd, ybar, gvb)
+            _, bwd = timed(backward, 20, 3, torch)
+            # the pair operator: A's values also in A^T's slot order
+            _, vt_pair = op.initial_values_pair()
+            gvt = torch.empty_like(vt_pair)
+            _, adj_pair = timed(lambda: op.adj.gT.matvec(ybar, gxb, vals=vt_pair), 20, 3, torch)
+
+            def backward_pair():
+                op.fwd.grad_vals(xd, ybar, gvb)
+                op.adj.adjoint_grad(ybar, gxb, xd, gvt, vals_t=vt_pair)
+            _, bwd_pair = timed(backward_pair, 20, 3, torch)
+        out[name] = {"fd_directions": 64, "h": h, "fd_worst_rel_err": worst, "tol": tol,
+                     "placement": placement, "forward_us": round(fwd * 1e6, 2),
+                     "adjoint_x_us": round(adj_pair * 1e6, 2),
+                     "adjoint_vs_forward": round(adj_pair / fwd, 3),
+                     "backward_pair_us": round(bwd_pair * 1e6, 2),
+                     "backward_pair_vs_forward": round(bwd_pair / fwd, 3),
+                     "adjoint_x_slot_map_us": round(adj * 1e6, 2),
+                     "adjoint_slot_map_vs_forward": round(adj / fwd, 3),
+                     "backward_slot_map_us": round(bwd * 1e6, 2),
+                     "backward_slot_map_vs_forward": round(bwd / fwd, 3)}
+        log(f"[bench] DIFFERENTIATION {name}: {out[name]}")
+        del op, v, x, ybar, gv, gx
+        torch.cuda.empty_cache()
+    return out
+
+
 def check_precision(out, pending):
     """Max per-row scaled error of every precision result against the FP64
     oracle (cf_CSR) of the unrounded matrix; returns the names over their
@@ -789,6 +872,9 @@ def main():
     precision, prec_pending = None, []
     if not args.quick and not args.no_precision:
         precision, prec_pending = bench_precision(eng, torch, log)
+    differentiation = None
+    if not args.quick:
+        differentiation = bench_differentiation(eng, m, torch, log)
 
     # ---- CPU baseline + correctness (oracle as the checker) -------------------
     cpu, over = None, None
@@ -801,6 +887,9 @@ def main():
                 over.append("batch")
         if precision is not None:
             over += check_precision(precision, prec_pending)
+        for name, r in (differentiation or {}).items():
+            if not r["fd_worst_rel_err"] < r["tol"]:
+                over.append(f"differentiation:{name}")
         if over:
             log(f"[bench] OVER TOLERANCE: {over}")
         cpu = cpu_leg(m, x, a1, hosts, head_bytes, args.workload, args.quick)
@@ -851,6 +940,7 @@ def main():
         "formats": formats,
         "batch": batch,
         "precision": precision,
+        "differentiation": differentiation,
     }
     print(json.dumps(out), flush=True)
 
