@@ -238,6 +238,61 @@ def bench_precision(eng, torch, log, names=PRECISION_WORKLOADS):
     return out, pending
 
 
+def bench_reference_case(eng, torch, log):
+    """The reference-sized case: the 2-D Laplacian on 256 x 256 (n = 65,536),
+    x ~ U[-1, 1] seed 0, through the ordered CSR product and the embedded AXT
+    product at tile heights 1 and 4 (FP64).  Returns the timings and the
+    outputs for the checker: CSR bit for bit against cf_CSR, AXT bit for bit
+    against the CPU restatement of the reference kernel and within 1e-12 of
+    cf_CSR.  Correctness only: the matrix (~9 MB) stays in cache."""
+    from int_spmv_amd import device as D
+    from int_spmv_amd import host as H
+    from int_spmv_amd.workloads import laplace_256
+    m, x = laplace_256()
+    n = m.nrows
+    xd = eng.upload(x)
+    out, ys, hosts = {"n": n, "nnz": m.nnz}, {}, {}
+    dc = D.DeviceCSR(eng, m)
+    y = torch.full((n,), float("nan"), dtype=torch.float64, device=eng.torch_device)
+    _, e = timed(lambda: dc.cf_csr(xd, y), 10, 2, torch)
+    out["CSR_ORDERED_F64"] = {"us": round(e * 1e6, 2)}
+    torch.cuda.synchronize()
+    ys["CSR_ORDERED_F64"] = y.cpu().numpy()
+    for th in (1, 4):
+        a = H.AXT(m, x, th=th)
+        d = D.DeviceAXT(eng, a)
+        y = torch.full((n,), float("nan"), dtype=torch.float64, device=eng.torch_device)
+        _, e = timed(lambda: d.int_axt(y), 10, 2, torch)
+        out[f"AXT_NOC_H{th}_F64"] = {"us": round(e * 1e6, 2)}
+        torch.cuda.synchronize()
+        ys[f"AXT_NOC_H{th}_F64"] = y.cpu().numpy()
+        hosts[f"AXT_NOC_H{th}_F64"] = a
+        del d
+    log(f"[bench] REFERENCE CASE laplace_256: {out}")
+    return out, (m, x, ys, hosts)
+
+
+def check_reference_case(out, pending):
+    """cf_CSR's bits for the ordered CSR product; the reference AXT kernel's
+    bits (its CPU restatement) and 1e-12 of cf_CSR for the AXT products.
+    Returns the names that fail."""
+    import numpy as np
+
+    import oracle as O
+    m, x, ys, hosts = pending
+    yref = O.cf_csr(m.row, m.col, m.val, x)
+    bad = []
+    for name, y in ys.items():
+        f = out[name]
+        f["max_scaled_err_vs_fp64_oracle"] = O.scaled_err(m.row, m.col, m.val, x, yref, y)[0]
+        want = yref if name.startswith("CSR") else O.int_axt(hosts[name])
+        f["bitwise_vs_" + ("cf_csr" if name.startswith("CSR") else "reference_kernel")] = bool(
+            np.array_equal(y, want))
+        if not (np.array_equal(y, want) and f["max_scaled_err_vs_fp64_oracle"] <= 1e-12):
+            bad.append(f"laplace_256:{name}")
+    return bad
+
+
 def bench_differentiation(eng, m, torch, log):
     """L(v, x) = y_bar^T A(v) x on the benchmark matrix through the autograd
     operator (column map, tile height 1), FP64 and FP32: the largest relative
@@ -872,9 +927,10 @@ def main():
     precision, prec_pending = None, []
     if not args.quick and not args.no_precision:
         precision, prec_pending = bench_precision(eng, torch, log)
-    differentiation = None
+    differentiation, reference_case, ref_pending = None, None, None
     if not args.quick:
         differentiation = bench_differentiation(eng, m, torch, log)
+        reference_case, ref_pending = bench_reference_case(eng, torch, log)
 
     # ---- CPU baseline + correctness (oracle as the checker) -------------------
     cpu, over = None, None
@@ -887,6 +943,8 @@ def main():
                 over.append("batch")
         if precision is not None:
             over += check_precision(precision, prec_pending)
+        if reference_case is not None:
+            over += check_reference_case(reference_case, ref_pending)
         for name, r in (differentiation or {}).items():
             if not r["fd_worst_rel_err"] < r["tol"]:
                 over.append(f"differentiation:{name}")
@@ -934,13 +992,16 @@ def main():
                      "kernel": "axt_h1_kernel<double,double,double,false> (+ split fix-up)"},
         "errors": ({"checked": len(formats_checked(formats)) + sum(
                         "max_scaled_err_vs_fp64_oracle" in f for w in (precision or {}).values()
-                        for f in w["formats"].values()), "over_tolerance": over}
+                        for f in w["formats"].values()) + sum(
+                        isinstance(f, dict) and "max_scaled_err_vs_fp64_oracle" in f
+                        for f in (reference_case or {}).values()), "over_tolerance": over}
                    if over is not None else None),
         "cpu_baseline": cpu,
         "formats": formats,
         "batch": batch,
         "precision": precision,
         "differentiation": differentiation,
+        "reference_case": reference_case,
     }
     print(json.dumps(out), flush=True)
 
