@@ -1135,7 +1135,8 @@ extern "C" int intspmv_axt_plan_grouped(intspmv_ctx *ctx, const intspmv_axt_desc
                                         const uint32_t *con_host, uint32_t ngroups,
                                         const uint32_t *group_rows, intspmv_plan **plan)
 {
-    if (ngroups == 0 || !desc || desc->layout != INTSPMV_AXT_COLMAP) return INTSPMV_ERR_ARG;
+    if (ngroups == 0 || !desc || desc->layout != INTSPMV_AXT_COLMAP || desc->mode != INTSPMV_NOC)
+        return INTSPMV_ERR_ARG;
     return axt_plan_impl(ctx, desc, con_host, ngroups, group_rows, plan);
 }
 
