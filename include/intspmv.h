@@ -169,6 +169,18 @@ int intspmv_axt_plan(intspmv_ctx *ctx, const intspmv_axt_desc *desc, const uint3
 int intspmv_axt_plan_grouped(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
                              const uint32_t *con_host, uint32_t ngroups,
                              const uint32_t *group_rows, intspmv_plan **plan);
+/* The autotuner's placement choice behind the C ABI: builds both column-map
+ * plans of a matrix (intspmv_axt_plan, and intspmv_axt_plan_grouped with eight
+ * groups of equal tiles), times spmv_axt_gather with k right-hand sides under
+ * each (reps launches after one untimed, on the context's stream, on a zero X
+ * it allocates and frees) and returns the faster plan in *plan.  *grouped = 1
+ * when that is the grouped plan; the two times in microseconds per product go
+ * to us_plain / us_grouped (each may be NULL).  The plans give the same bits,
+ * so the choice changes only the time.  No reference counterpart. */
+int intspmv_axt_plan_tuned(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
+                           const uint32_t *con_host, const void *aval, const uint32_t *acol,
+                           const uint32_t *sec, uint32_t k, uint32_t reps, intspmv_plan **plan,
+                           int *grouped, float *us_plain, float *us_grouped);
 /* Embedded-x product.  INTERLEAVED: ax = the reference array, xplane = NULL.
  * PLANAR: ax = value plane, xplane = x plane.  y has the accumulation type
  * (double for F64 values, float otherwise). */

@@ -380,6 +380,75 @@ extern "C" int spmv_axt_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const
     return INTSPMV_ERR_UNSUPPORTED;
 }
 
+// Times `reps` products of k right-hand sides under a plan (after one
+// untimed), on the context's stream; microseconds per product in *us.
+static int time_gather(intspmv_ctx *ctx, const intspmv_plan *plan, const void *aval,
+                       const uint32_t *acol, const uint32_t *sec, const void *x, void *y,
+                       uint32_t k, uint32_t reps, float *us)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int st = e == hipSuccess ? spmv_axt_gather(ctx, plan, aval, acol, sec, x, k, y, k, k, 0)
+                             : INTSPMV_ERR_HIP;
+    if (!st && hipEventRecord(e0, ctx->stream) != hipSuccess) st = INTSPMV_ERR_HIP;
+    for (uint32_t r = 0; r < reps && !st; r++)
+        st = spmv_axt_gather(ctx, plan, aval, acol, sec, x, k, y, k, k, 0);
+    float ms = 0.0f;
+    if (!st && (hipEventRecord(e1, ctx->stream) != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = INTSPMV_ERR_HIP;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (!st) *us = ms * 1e3f / (float)reps;
+    return st;
+}
+
+extern "C" int intspmv_axt_plan_tuned(intspmv_ctx *ctx, const intspmv_axt_desc *desc,
+                                      const uint32_t *con_host, const void *aval,
+                                      const uint32_t *acol, const uint32_t *sec, uint32_t k,
+                                      uint32_t reps, intspmv_plan **plan, int *grouped,
+                                      float *us_plain, float *us_grouped)
+{
+    if (!plan || !desc || !ctx || k == 0 || k > PARTIAL_STRIDE || reps == 0 ||
+        desc->layout != INTSPMV_AXT_COLMAP || desc->mode != INTSPMV_NOC)
+        return INTSPMV_ERR_ARG;
+    *plan = nullptr;
+    intspmv_plan *pp = nullptr, *pg = nullptr;
+    int st = intspmv_axt_plan(ctx, desc, con_host, &pp);
+    if (!st) st = intspmv_axt_plan_grouped(ctx, desc, con_host, NUM_XCD, nullptr, &pg);
+    if (st) {
+        intspmv_plan_destroy(pp);
+        return st;
+    }
+    // scratch X (zeros) and Y, n x k of the value type's accumulation type
+    const size_t esz = desc->x_type == INTSPMV_F64 ? 8 : 4;
+    const size_t bytes = ((size_t)desc->nrows * k + 1) * esz;
+    void *x = nullptr, *y = nullptr;
+    hipError_t e = hipMalloc(&x, bytes);
+    if (e == hipSuccess) e = hipMalloc(&y, bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(x, 0, bytes, ctx->stream);
+    float tp = 0.0f, tg = 0.0f;
+    st = e == hipSuccess ? INTSPMV_OK : INTSPMV_ERR_NOMEM;
+    if (!st) st = time_gather(ctx, pp, aval, acol, sec, x, y, k, reps, &tp);
+    if (!st) st = time_gather(ctx, pg, aval, acol, sec, x, y, k, reps, &tg);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+    if (st) {
+        intspmv_plan_destroy(pp);
+        intspmv_plan_destroy(pg);
+        return st;
+    }
+    const bool g = tg < tp;
+    intspmv_plan_destroy(g ? pp : pg);
+    *plan = g ? pg : pp;
+    if (grouped) *grouped = g;
+    if (us_plain) *us_plain = tp;
+    if (us_grouped) *us_grouped = tg;
+    return INTSPMV_OK;
+}
+
 extern "C" int spmv_axt_gather_permuted(intspmv_ctx *ctx, const intspmv_plan *plan,
                                         const void *vals, const uint32_t *vperm,
                                         const uint32_t *acol, const uint32_t *sec, const void *x,

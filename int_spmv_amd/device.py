@@ -76,6 +76,9 @@ SIGNATURES = [
     ("intspmv_axt_plan", C.c_int, [_vp, C.POINTER(AxtDesc), _vp, C.POINTER(_vp)]),
     ("intspmv_axt_plan_grouped", C.c_int,
      [_vp, C.POINTER(AxtDesc), _vp, _u32, _vp, C.POINTER(_vp)]),
+    ("intspmv_axt_plan_tuned", C.c_int,
+     [_vp, C.POINTER(AxtDesc), _vp, _vp, _vp, _vp, _u32, _u32, C.POINTER(_vp),
+      C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("spmv_axt", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("spmv_axt_gather", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32, _u32]),
     ("spmv_axt_gather_permuted", C.c_int,
@@ -562,6 +565,21 @@ class DeviceAXTGather:
             self.plans[key] = self._make_plan(placement)
         self.plan = self.plans[key]
         self.placement = key
+
+    def tune_placement_c(self, k: int = 1, reps: int = 10) -> dict:
+        """The same choice made by the library (intspmv_axt_plan_tuned): keeps
+        the faster of the two plans as the "tuned" placement."""
+        h, g = _vp(), C.c_int()
+        tp, tg = C.c_float(), C.c_float()
+        con = np.ascontiguousarray(self.a.con)
+        self.eng._check(lib().intspmv_axt_plan_tuned(
+            self.eng.h, C.byref(self.desc), _np_ptr(con), _dptr(self.aval), _dptr(self.acol),
+            _dptr(self.sec), k, reps, C.byref(h), C.byref(g), C.byref(tp), C.byref(tg)),
+            "axt_plan_tuned")
+        self.plans["tuned"] = _Plan(self.eng, h)
+        self.plan, self.placement = self.plans["tuned"], "tuned"
+        return dict(grouped=bool(g.value), plain_us=round(tp.value, 2),
+                    xcd_us=round(tg.value, 2))
 
     def tune_placement(self, k: int = 1, reps: int = 10) -> dict:
         """Times the product with k right-hand sides under the "plain" and

@@ -130,3 +130,22 @@ def test_csr_grouped_plan_same_bits(eng, name, m):
     yref = O.cf_csr(m.row, m.col, m.val, x)
     err, row = O.scaled_err(m.row, m.col, m.val, x, yref, out["xcd"].numpy())
     assert err <= 1e-12, f"{name}: {err} at row {row}"
+
+
+@pytest.mark.parametrize("vt,dt", [(D.F64, torch.float64), (D.F32, torch.float32)])
+def test_plan_tuned_in_c(eng, vt, dt):
+    """intspmv_axt_plan_tuned keeps the faster of the two plans (by its own
+    timings) and the kept plan gives the ungrouped plan's bits."""
+    m = H.CSR.powerlaw(n=20000, target_nnz=600000, maxlen=8192, band_frac=0.8, band_half=500,
+                       seed=4)
+    n = m.nrows
+    g = D.DeviceAXTGather(eng, H.AXT(m, H.uniform(n, -1.0, 1.0, seed=2), th=1, want_acol=True),
+                          vt, vt)
+    X = torch.rand(n * 3, dtype=dt, device="cuda") * 2 - 1
+    ref = _run(g, X, 3, dt, n)
+    r = g.tune_placement_c(k=3, reps=3)
+    assert g.placement == "tuned" and r["plain_us"] > 0 and r["xcd_us"] > 0
+    assert r["grouped"] == (r["xcd_us"] < r["plain_us"])
+    assert torch.equal(_run(g, X, 3, dt, n), ref)
+    with pytest.raises(D.IntSpmvError):  # k outside 1..32 is refused
+        g.tune_placement_c(k=33, reps=1)
