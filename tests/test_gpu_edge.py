@@ -54,6 +54,13 @@ def test_every_format_on_degenerate_matrices(engine, name, m):
     dc.cf_csr(xd, y)
     got["csr_ordered"] = _host(y, n)
     y = _nan(engine, n)
+    dc.spmv_exact(xd, y)
+    got["csr_exact_plan"] = _host(y, n)
+    dc.set_placement("xcd")
+    y = _nan(engine, n)
+    dc.spmv(xd, y)
+    got["csr_grouped_plan"] = _host(y, n)
+    y = _nan(engine, n)
     D.DeviceAXC(engine, H.AXC(m, x)).int_axc(y)
     got["axc"] = _host(y, n)
     k = H.K1(m)
@@ -79,6 +86,12 @@ def test_every_format_on_degenerate_matrices(engine, name, m):
     y = _nan(engine, n)
     g.matvec(xd, y)
     got["colmap"] = _host(y, n)
+    for pl in ("xcd", np.array([0, n], np.uint32)):  # grouped plans: eight even groups, one group
+        g.set_placement(pl)
+        y = _nan(engine, n)
+        g.matvec(xd, y)
+        got[f"colmap_grouped_{len(pl) if not isinstance(pl, str) else pl}"] = _host(y, n)
+    g.set_placement("plain")
     for name_f, got_y in got.items():
         assert bitwise_equal(got_y, yr), (name, name_f, got_y, yr)
     # multi-RHS, adjoint and a batch of the matrix
