@@ -572,31 +572,32 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
     const BlockDesc d = blocks[bid];
     const uint32_t nu = d.uend - d.ubeg;
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
-    // long rows: lane-strided partials; FP64 an fma chain, FP32 a
-    // compensated (Neumaier) sum of the products
-    AT acc = 0;
-    LaneSum<AT> cacc;
-    // full-line mapping: 8 whole tiles per load, lane part p = l%8; values of
-    // slots 2p, 2p+1 (p < 4) meet their x through a DPP row shift
-    const uint32_t p = lane & 7, Cq = 2 * (p & 3);
-    for (uint32_t Tb = T0; Tb < T1; Tb += 8 * COM_U) {
-        LoadT<VT> v0[COM_U], v1[COM_U];
-        u2v hw[COM_U];
+    // full-line mapping: 8 whole tiles per load, lane part p = l%8 reads
+    // doubles 2p, 2p+1 of its tile (values of slots 2p, 2p+1 for p < 4, the
+    // x of slots 2p-8, 2p-7 for p >= 4)
+    const uint32_t p = lane & 7;
+    if (d.k != 0) {
+        // long rows: lane-strided partials of slots 2p, 2p+1 (p < 4), whose x
+        // a DPP row shift brings; FP64 an fma chain, FP32 a compensated
+        // (Neumaier) sum of the products
+        AT acc = 0;
+        LaneSum<AT> cacc;
+        const uint32_t Cq = 2 * (p & 3);
+        for (uint32_t Tb = T0; Tb < T1; Tb += 8 * COM_U) {
+            LoadT<VT> v0[COM_U], v1[COM_U];
 #pragma unroll
-        for (int u = 0; u < COM_U; u++) {
-            const uint32_t Tr = Tb + 8 * u + lane / 8;
-            const uint32_t T = Tr < T1 ? Tr : T1 - 1;
-            Pair<VT>::ld(ax + (size_t)T * 16 + 2 * p, v0[u], v1[u]);
-            if (d.k == 0) hw[u] = ntload(reinterpret_cast<const u2v *>(sec + 8 * (size_t)T + Cq));
-        }
+            for (int u = 0; u < COM_U; u++) {
+                const uint32_t Tr = Tb + 8 * u + lane / 8;
+                const uint32_t T = Tr < T1 ? Tr : T1 - 1;
+                Pair<VT>::ld(ax + (size_t)T * 16 + 2 * p, v0[u], v1[u]);
+            }
 #pragma unroll
-        for (int u = 0; u < COM_U; u++) {
-            const uint32_t Tr = Tb + 8 * u + lane / 8, u0 = 8 * Tr + Cq;
-            const AT x0 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v0[u]);
-            const AT x1 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v1[u]);
-            const bool ok0 = p < 4 && Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
-            const bool ok1 = p < 4 && Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
-            if (d.k != 0) {
+            for (int u = 0; u < COM_U; u++) {
+                const uint32_t Tr = Tb + 8 * u + lane / 8, u0 = 8 * Tr + Cq;
+                const AT x0 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v0[u]);
+                const AT x1 = (AT)dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v1[u]);
+                const bool ok0 = p < 4 && Tr < T1 && u0 >= d.ubeg && u0 < d.uend;
+                const bool ok1 = p < 4 && Tr < T1 && u0 + 1 >= d.ubeg && u0 + 1 < d.uend;
                 if constexpr (sizeof(AT) == 4) {
                     if (ok0) cacc.add((AT)v0[u] * x0);
                     if (ok1) cacc.add((AT)v1[u] * x1);
@@ -604,24 +605,46 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
                     if (ok0) acc = fma_acc<AT>((AT)v0[u], x0, acc);
                     if (ok1) acc = fma_acc<AT>((AT)v1[u], x1, acc);
                 }
-                continue;
-            }
-            if (ok0) {
-                const bool h = hw[u][0] != 0 || u0 % bs == 0;
-                s_vx[w][u0 - d.ubeg] = {(AT)v0[u], x0};
-                s_key[w][u0 - d.ubeg] = h ? hw[u][0] >> lg : COM_NOHEAD;
-            }
-            if (ok1) {
-                const bool h = hw[u][1] != 0 || (u0 + 1) % bs == 0;
-                s_vx[w][u0 + 1 - d.ubeg] = {(AT)v1[u], x1};
-                s_key[w][u0 + 1 - d.ubeg] = h ? hw[u][1] >> lg : COM_NOHEAD;
             }
         }
-    }
-    if (d.k != 0) {
         if constexpr (sizeof(AT) == 4) acc = cacc.value();
         finish_long<AT>(wave_sum<AT>(acc), d, bid, y, nrows, partial);
         return;
+    }
+    // Normal block: at most COM_UMAX slots, all loaded in one step.  Each
+    // lane stages one slot: part p < 4 slot 2p (its own value, the x of part
+    // p + 4 by a left row shift), part p >= 4 slot 2p - 7 (its own x, the
+    // value of part p - 4 by a right row shift).  bs is a power of two.  A
+    // block without slots (T0 == T1, e.g. empty rows at the start) loads
+    // nothing: its clamp tile T1 - 1 may not exist.
+    if (T0 < T1) {
+        const uint32_t sl = p < 4 ? 2 * p : 2 * p - 7, bm = bs - 1;
+        LoadT<VT> v0[COM_U], v1[COM_U];
+        uint32_t hw[COM_U];
+#pragma unroll
+        for (int u = 0; u < COM_U; u++) {
+            const uint32_t Tr = T0 + 8 * u + lane / 8;
+            const uint32_t T = Tr < T1 ? Tr : T1 - 1;
+            Pair<VT>::ld(ax + (size_t)T * 16 + 2 * p, v0[u], v1[u]);
+            hw[u] = ntload(sec + 8 * (size_t)T + sl);
+        }
+        // all loads issue before any of their uses (the scheduler would hoist
+        // the first row shift between them, a wait for the first load)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < COM_U; u++) {
+            // slots past the block (clamped tiles included) give i >= nu
+            const uint32_t s = 8 * (T0 + 8 * u + lane / 8) + sl, i = s - d.ubeg;
+            // keeps the head-word load with the others (the compiler would
+            // otherwise sink it into the store below, one more round trip)
+            asm volatile("" : "+v"(hw[u]));
+            const LoadT<VT> xl = dpp_any<LoadT<VT>, DPP_ROW_SHL4>(v0[u]);
+            const LoadT<VT> vr = dpp_any<LoadT<VT>, DPP_ROW_SHR4>(v1[u]);
+            if (i < nu) {
+                s_vx[w][i] = {(AT)(p < 4 ? v0[u] : vr), (AT)(p < 4 ? xl : v1[u])};
+                s_key[w][i] = (hw[u] != 0 || (s & bm) == 0) ? hw[u] >> lg : COM_NOHEAD;
+            }
+        }
     }
     wave_lds_sync();
     // Rows: a head whose row differs from the previous head's starts a row

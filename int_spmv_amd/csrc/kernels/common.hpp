@@ -119,6 +119,7 @@ constexpr int DPP_QUAD_XOR2 = 0x4E; // quad_perm [2,3,0,1]
 constexpr int DPP_ROW_SHL4 = 0x104; // lane i reads lane i+4 of its row of 16
 constexpr int DPP_ROW_SHL2 = 0x102; // lane i reads lane i+2 of its row of 16
 constexpr int DPP_ROW_SHL8 = 0x108; // lane i reads lane i+8 of its row of 16
+constexpr int DPP_ROW_SHR4 = 0x114; // lane i reads lane i-4 of its row of 16
 
 __device__ __forceinline__ double shfl_xor_f64(double v, int m) { return __shfl_xor(v, m, WAVE); }
 __device__ __forceinline__ float shfl_xor_f32(float v, int m) { return __shfl_xor(v, m, WAVE); }
