@@ -218,6 +218,7 @@ __device__ __forceinline__ AT h1_long(const VT *av, const XT *xb, const BlockDes
                 size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
                 v[j] = ntload(reinterpret_cast<const d2v *>(av + T * 16 + 2 * p));
             }
+            loads_issued();
 #pragma unroll
             for (int j = 0; j < 2 * UL; j++) {
                 double x0 = dpp_f64<DPP_ROW_SHL4>(v[j][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[j][1]);
@@ -239,6 +240,7 @@ __device__ __forceinline__ AT h1_long(const VT *av, const XT *xb, const BlockDes
                 size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
                 v[j] = f4load(reinterpret_cast<const float *>(av) + T * 16 + 4 * (p % 4));
             }
+            loads_issued();
 #pragma unroll
             for (int j = 0; j < UL; j++) {
                 float4 xx = f4dpp<DPP_ROW_SHL2>(v[j]);
@@ -287,27 +289,34 @@ __device__ __forceinline__ AT h1_long(const VT *av, const XT *xb, const BlockDes
     return wave_sum<AT>(acc.value());
 }
 
-// Stages a normal block's sec words (one per unit, nu <= UMAX) in LDS with
-// one lane per unit, so the keys cost one or two coalesced loads per block
-// and no registers across the value loads.
+// A normal block's sec words (one per unit, nu <= UMAX), one lane per unit:
+// load() issues one or two coalesced loads, store() puts them in LDS.  The
+// kernels load the keys before their value loads and store them after the
+// values are consumed, so that no wait for the keys holds the value loads
+// back (the keys arrive first: loads return in order).
 template <uint32_t UMAX = AXT_UMAX>
-__device__ __forceinline__ void stage_keys(const uint32_t *sec, const BlockDesc &d, uint32_t nu,
-                                           uint32_t *sk)
-{
+struct BlockKeys {
     static_assert(UMAX % WAVE == 0, "key staging");
-    if (nu == 0) return;
-    const int lane = lane_id();
     uint32_t kk[UMAX / WAVE];
-    kk[0] = ntload(sec + d.ubeg + (lane < (int)nu ? lane : nu - 1));
+    __device__ __forceinline__ void load(const uint32_t *sec, const BlockDesc &d, uint32_t nu)
+    {
+        if (nu == 0) return;
+        const int lane = lane_id();
+        kk[0] = ntload(sec + d.ubeg + (lane < (int)nu ? lane : nu - 1));
 #pragma unroll
-    for (int j = 1; j < (int)(UMAX / WAVE); j++) {
-        const uint32_t t = j * WAVE + lane;
-        if (j * WAVE < (int)nu) kk[j] = ntload(sec + d.ubeg + (t < nu ? t : nu - 1)); // uniform
+        for (int j = 1; j < (int)(UMAX / WAVE); j++) {
+            const uint32_t t = j * WAVE + lane;
+            if (j * WAVE < (int)nu) kk[j] = ntload(sec + d.ubeg + (t < nu ? t : nu - 1)); // uniform
+        }
     }
+    __device__ __forceinline__ void store(uint32_t nu, uint32_t *sk) const
+    {
+        const int lane = lane_id();
 #pragma unroll
-    for (int j = 0; j < (int)(UMAX / WAVE); j++)
-        if (j * WAVE + lane < nu) sk[j * WAVE + lane] = kk[j];
-}
+        for (int j = 0; j < (int)(UMAX / WAVE); j++)
+            if (j * WAVE + lane < nu) sk[j * WAVE + lane] = kk[j];
+    }
+};
 
 #ifdef ISPV_WAVES_PER_EU
 #define ISPV_OCC __attribute__((amdgpu_waves_per_eu(ISPV_WAVES_PER_EU)))
@@ -339,30 +348,43 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
         // 2p,2p+1 (p<4) or x 2(p-4),2(p-4)+1 (p>=4); one DPP row shift brings
         // the x pair to the value lanes, whose quads then form the tree.  A
         // 64-tile block is in flight at once (8 loads).
+        // Blocks of up to AXT_UMAX tiles (plans of matrices with many empty
+        // rows) have all their rounds of U loads in flight together; a round
+        // past the block's tiles (every round of a block of empty rows, whose
+        // clamp tile nu - 1 does not exist) loads nothing.
         const uint32_t p = lane & 7;
-        constexpr int U = ISPV_H1_F64_UMAX / 8;
-        stage_keys(sec, d, nu, s_key[w]);
-        for (uint32_t base = 0; base < nu; base += 8 * U) {
-            d2v v[U];
+        constexpr int U = 8, NR = AXT_UMAX / (8 * U);
+        BlockKeys<AXT_UMAX> keys;
+        keys.load(sec, d, nu);
+        d2v v[NR][U];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (nu <= 8 * U * r) break; // uniform
 #pragma unroll
             for (int j = 0; j < U; j++) {
-                uint32_t t = base + j * 8 + lane / 8;
+                uint32_t t = 8 * U * r + j * 8 + lane / 8;
                 size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
-                v[j] = ntload(reinterpret_cast<const d2v *>(av + T * 16 + 2 * p));
+                v[r][j] = ntload(reinterpret_cast<const d2v *>(av + T * 16 + 2 * p));
             }
+        }
+        loads_issued();
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (nu <= 8 * U * r) break; // uniform
 #pragma unroll
             for (int j = 0; j < U; j++) {
-                const bool ok = base + j * 8 + lane / 8 < nu && p < 4;
-                double x0 = dpp_f64<DPP_ROW_SHL4>(v[j][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[j][1]);
-                double p0 = ok ? v[j][0] * x0 : 0.0, p1 = ok ? v[j][1] * x1 : 0.0;
+                uint32_t t = 8 * U * r + j * 8 + lane / 8;
+                const bool ok = t < nu && p < 4;
+                double x0 = dpp_f64<DPP_ROW_SHL4>(v[r][j][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[r][j][1]);
+                double p0 = ok ? v[r][j][0] * x0 : 0.0, p1 = ok ? v[r][j][1] * x1 : 0.0;
                 p0 = p0 + dpp_f64<DPP_QUAD_XOR2>(p0);
                 p1 = p1 + dpp_f64<DPP_QUAD_XOR2>(p1);
                 p0 = p0 + dpp_f64<DPP_QUAD_XOR1>(p0);
                 p1 = p1 + dpp_f64<DPP_QUAD_XOR1>(p1);
-                uint32_t t = base + j * 8 + lane / 8;
                 if (t < nu && p == 0) s_val[w][t] = (AT)(p0 + p1);
             }
         }
+        keys.store(nu, s_key[w]);
         wave_lds_sync();
         reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
         return;
@@ -371,7 +393,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
         // FP32 interleaved, two 64-byte tiles per 128-byte line (see h1_long)
         const uint32_t p = lane & 7;
         constexpr int U = 2 * ISPV_H1_STEPS;
-        stage_keys(sec, d, nu, s_key[w]);
+        BlockKeys<AXT_UMAX> keys;
+        keys.load(sec, d, nu);
         for (uint32_t base = 0; base < nu; base += 16 * U) {
             float4 v[U];
 #pragma unroll
@@ -380,6 +403,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
                 size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
                 v[j] = f4load(reinterpret_cast<const float *>(av) + T * 16 + 4 * (p % 4));
             }
+            loads_issued();
 #pragma unroll
             for (int j = 0; j < U; j++) {
                 uint32_t t = base + j * 16 + 2 * (lane / 8) + p / 4;
@@ -389,6 +413,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
                 if (t < nu && p % 4 == 0) s_val[w][t] = (AT)ps;
             }
         }
+        keys.store(nu, s_key[w]);
         wave_lds_sync();
         reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
         return;
@@ -396,7 +421,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
     if constexpr (PLANAR) {
         // planar storage: one lane per tile
         constexpr int U = ISPV_H1_STEPS;
-        stage_keys(sec, d, nu, s_key[w]);
+        BlockKeys<AXT_UMAX> keys;
+        keys.load(sec, d, nu);
         for (uint32_t base = 0; base < nu; base += 64 * U) {
             AT ts[U];
 #pragma unroll
@@ -411,6 +437,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
                 if (t < nu) s_val[w][t] = ts[j];
             }
         }
+        keys.store(nu, s_key[w]);
         wave_lds_sync();
         reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
         return;
@@ -431,6 +458,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
             Pair<XT>::ld(xb + xidx<PLANAR>(T, 1, 0, C), x0[j], x1[j]);
             key[j] = ntload(sec + T);
         }
+        loads_issued();
 #pragma unroll
         for (int j = 0; j < U; j++) {
             const bool ok = base + j * 16 + lane / 4 < nu;
@@ -503,7 +531,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
     const XT *xb = PLANAR ? axv : (const XT *)av;
     LaneSum<AT> lacc; // long rows
-    if (d.k == 0) stage_keys(sec, d, nu, s_key[w]);
+    BlockKeys<AXT_UMAX> keys;
+    if (d.k == 0) keys.load(sec, d, nu);
     for (uint32_t Tb = T0; Tb < T1; Tb += 16) {
         const uint32_t Tr = Tb + lane / 4, u0 = 8 * Tr + C;
         const bool ok0 = u0 >= d.ubeg && u0 < d.uend;
@@ -525,6 +554,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_kernel(
         finish_long<AT>(wave_sum<AT>(lacc.value()), d, bid, y, nrows, partial);
         return;
     }
+    keys.store(nu, s_key[w]);
     wave_lds_sync();
     reduce_keyed<AT>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
 }
@@ -628,9 +658,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_com_kernel(
             Pair<VT>::ld(ax + (size_t)T * 16 + 2 * p, v0[u], v1[u]);
             hw[u] = ntload(sec + 8 * (size_t)T + sl);
         }
-        // all loads issue before any of their uses (the scheduler would hoist
-        // the first row shift between them, a wait for the first load)
-        __builtin_amdgcn_sched_barrier(0);
+        loads_issued();
 #pragma unroll
         for (int u = 0; u < COM_U; u++) {
             // slots past the block (clamped tiles included) give i >= nu
@@ -727,7 +755,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
     const uint32_t nu = d.uend - d.ubeg, p = lane & 7, C = 2 * (p & 3);
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
     double lacc = 0.0; // long rows
-    if (d.k == 0) stage_keys<UM>(sec, d, nu, s_key[w]); // unit u = column 8T+C has key sec[u]
+    BlockKeys<UM> keys; // unit u = column 8T+C has key sec[u]
+    if (d.k == 0) keys.load(sec, d, nu);
     // G groups of 8 tiles per step, so that a normal block is in flight at
     // once: th = 2: 32 tiles; th = 4: 16 tiles (th = 8, 16: 8 tiles per step)
     constexpr int G = TH == 2 ? (int)(TH2_UMAX / 64) : (TH == 0 || TH >= 8) ? 1 : 2;
@@ -743,6 +772,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
                 for (int R = 0; R < TH; R++)
                     v[g][R] = ntload(reinterpret_cast<const d2v *>(ax + (T * TH + R) * 16 + 2 * p));
             }
+            loads_issued();
 #pragma unroll
             for (int g = 0; g < G; g++) {
                 a0[g] = 0.0;
@@ -767,6 +797,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
                 for (int r = 0; r < 8; r++)
                     if (TH > 0 || R0 + r < th) // whole chunks when TH (a multiple of 8) is known
                         v[r] = ntload(reinterpret_cast<const d2v *>(ax + (T * th + R0 + r) * 16 + 2 * p));
+                loads_issued();
 #pragma unroll
                 for (int r = 0; r < 8; r++) {
                     if (TH == 0 && R0 + r >= th) break;
@@ -794,6 +825,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl_kernel(
         finish_long<double>(wave_sum_f64(lacc), d, bid, y, nrows, partial);
         return;
     }
+    keys.store(nu, s_key[w]);
     wave_lds_sync();
     reduce_keyed<double>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
 }
@@ -822,7 +854,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl32_kernel(
     const uint32_t nu = d.uend - d.ubeg, p = lane & 7;
     const uint32_t T0 = d.ubeg / 8, T1 = (d.uend + 7) / 8;
     LaneSum<float> lacc; // long rows
-    if (d.k == 0) stage_keys<UM>(sec, d, nu, s_key[w]);
+    BlockKeys<UM> keys;
+    if (d.k == 0) keys.load(sec, d, nu);
     constexpr int L = TH / 2; // lines per tile
     // tile groups of 8 tiles per step: the whole block in flight (th 8: 8-tile blocks)
     constexpr int G = L >= 4 ? 1 : (int)(UM / 64);
@@ -836,6 +869,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl32_kernel(
             for (int i = 0; i < L; i++)
                 v[g][i] = f4load(ax + (T * TH + 2 * i + p / 4) * 16 + 4 * (p % 4));
         }
+        loads_issued();
 #pragma unroll
         for (int g = 0; g < G; g++) {
             float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -870,6 +904,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl32_kernel(
         finish_long<float>(wave_sum_f32(lacc.value()), d, bid, y, nrows, partial);
         return;
     }
+    keys.store(nu, s_key[w]);
     wave_lds_sync();
     reduce_keyed<float>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
 }
@@ -921,7 +956,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_planar_kernel(
     const size_t T0 = TH == 1 ? d.ubeg : d.ubeg / 8, T1 = TH == 1 ? d.uend : (d.uend + 7) / 8;
     const size_t H0 = T0 * 2 * TH, H1 = T1 * 2 * TH;
     LaneSum<float> lacc; // long rows
-    if (d.k == 0) stage_keys<UM>(sec, d, nu, s_key[w]);
+    BlockKeys<UM> keys;
+    if (d.k == 0) keys.load(sec, d, nu);
     // load steps of 64 half-brick-rows: th 2, 4 take their whole block at once
     constexpr int U = TH == 1 ? ISPV_PLANAR_STEPS
                     : (TH == 2 || TH == 4) ? (int)(UM * TH / 256) : ISPV_PLANAR_STEPS_TH;
@@ -970,6 +1006,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_planar_kernel(
         finish_long<float>(wave_sum_f32(lacc.value()), d, bid, y, nrows, partial);
         return;
     }
+    keys.store(nu, s_key[w]);
     wave_lds_sync();
     reduce_keyed<float>(s_val[w], s_key[w], s_seg[w], nu, d, bid, y, nrows, partial);
 }

@@ -86,6 +86,11 @@ typedef uint32_t u2v __attribute__((ext_vector_type(2)));
 template <typename T>
 __device__ __forceinline__ T ntload(const T *p) { return __builtin_nontemporal_load(p); }
 
+// Keeps the loads issued above the call ahead of every instruction below it.
+// The scheduler otherwise hoists an early consumer between them, and that
+// consumer's wait for the first load holds back the loads after it.
+__device__ __forceinline__ void loads_issued() { __builtin_amdgcn_sched_barrier(0); }
+
 // Orders this wave's LDS accesses before/after the call (no cross-wave sync).
 __device__ __forceinline__ void wave_lds_sync()
 {
