@@ -693,6 +693,9 @@ def main():
            {"beta": round(a1.fd.beta, 4), "plan": d1.plan.info()})
     keep("AXT_NOC_H1_F64", "y", y1, n)
     head_wall, head_ev = wall, ev
+    # what a plain read of as many bytes reaches on this device, back to back
+    # like the timed launches: the streaming ceiling beside the 8 TB/s peak
+    stream_bps = eng.stream_rate(int(head_bytes), 20)
     hosts = {}
     if args.quick:  # headline launches only (profiling)
         warm = cold = None
@@ -989,7 +992,9 @@ def main():
                      "cold_us_median": cold_us,
                      "cold_frac": (round(head_bytes / (cold_us * 1e-6) / 1e9 / M.HBM_PEAK_GBS, 4)
                                    if cold_us else None),
-                     "kernel": "axt_h1_kernel<double,double,double,false> (+ split fix-up)"},
+                     "kernel": "axt_h1_kernel<double,double,double,false> (+ split fix-up)",
+                     "stream_read_GBs": round(stream_bps / 1e9, 1),
+                     "frac_of_stream_read": round(achieved * 1e9 / stream_bps, 4)},
         "errors": ({"checked": len(formats_checked(formats)) + sum(
                         "max_scaled_err_vs_fp64_oracle" in f for w in (precision or {}).values()
                         for f in w["formats"].values()) + sum(

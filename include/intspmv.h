@@ -312,6 +312,14 @@ int spmv_axt_h1_noalloc(intspmv_ctx *ctx, uint32_t nrows, uint32_t tileN, const 
  * frees its own buffers; synchronises the stream. */
 int intspmv_gather_rate(intspmv_ctx *ctx, size_t span_bytes, uint32_t elem_bytes, size_t count,
                         uint32_t reps, double *gathers_per_s);
+/* Measurement utility, no reference counterpart: the rate (bytes per second)
+ * of a plain read of a `bytes`-long device buffer (16-byte nontemporal loads,
+ * whole lines per load instruction, each wave a contiguous 1, 2, 4 or 8 KB:
+ * the best of the four), timed over `reps` back-to-back launches on the
+ * context's stream.  It is the ceiling the streaming products (the
+ * embedded-x AXT, AXC) are measured against, beside the 8 TB/s peak.
+ * Allocates and frees its own buffer; synchronises the stream. */
+int intspmv_stream_rate(intspmv_ctx *ctx, size_t bytes, uint32_t reps, double *bytes_per_s);
 int intspmv_plan_destroy(intspmv_plan *plan);
 /* Host-only inspection of the work decomposition a plan uses for a row ->
  * unit pointer uptr[nrows+1]: blocks (8 words each: ubeg, uend, rbeg, rend,

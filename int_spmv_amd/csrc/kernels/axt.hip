@@ -353,7 +353,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) ISPV_OCC void axt_h1_kernel(
         // past the block's tiles (every round of a block of empty rows, whose
         // clamp tile nu - 1 does not exist) loads nothing.
         const uint32_t p = lane & 7;
-        constexpr int U = 8, NR = AXT_UMAX / (8 * U);
+        constexpr int U = ISPV_H1_F64_UMAX < 64 ? ISPV_H1_F64_UMAX / 8 : 8;
+        constexpr int NR = AXT_UMAX / (8 * U);
         BlockKeys<AXT_UMAX> keys;
         keys.load(sec, d, nu);
         d2v v[NR][U];

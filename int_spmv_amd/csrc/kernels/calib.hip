@@ -9,8 +9,13 @@
 // streams a 4-byte index (as acol is streamed) and gathers one element at it,
 // eight gathers in flight per thread, from a buffer of a given span.  A span
 // inside one XCD's L2 (4 MB) gives the L2 request ceiling, the span of a
-// product's x the ceiling for uniformly placed columns.  No reference
-// counterpart: a measurement utility for bench.py.
+// product's x the ceiling for uniformly placed columns.
+// intspmv_stream_rate times the other ceiling, that of the streaming products:
+// a plain read of a buffer of a given size with 16-byte nontemporal loads,
+// every load instruction covering whole lines (the best of 1 to 8 KB per wave).
+// No reference counterpart: measurement utilities for bench.py.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace ispv {
@@ -64,7 +69,87 @@ __global__ __launch_bounds__(256) void calib_gather_kernel(const uint32_t *__res
     out[t] = acc;
 }
 
+// Reads n16 16-byte words the way the streaming products read their blocks:
+// each wave one contiguous chunk of U 1-KB loads (64 lanes x 16 B), all in
+// flight, workgroups XCD-remapped so each XCD reads a contiguous range.  The
+// sum is stored only if it is NaN (never, for the zero-filled buffer), which
+// keeps the loads live without write traffic.
+template <int U>
+__global__ __launch_bounds__(256) void calib_stream_kernel(const d2v *__restrict__ buf, size_t n16,
+                                                           double *__restrict__ out)
+{
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const size_t chunk = (size_t)g * WAVES + threadIdx.x / WAVE;
+    const size_t base = chunk * (U * WAVE) + threadIdx.x % WAVE;
+    d2v v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const size_t j = base + (size_t)u * WAVE;
+        v[u] = j < n16 ? ntload(buf + j) : d2v{0.0, 0.0};
+    }
+    loads_issued();
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++) acc += v[u][0] + v[u][1];
+    if (acc != acc) out[chunk] = acc;
+}
+
 } // namespace ispv
+
+extern "C" int intspmv_stream_rate(intspmv_ctx *ctx, size_t bytes, uint32_t reps,
+                                   double *bytes_per_s)
+{
+    using namespace ispv;
+    if (!ctx || !bytes_per_s || reps == 0 || bytes < 16) return INTSPMV_ERR_ARG;
+    const size_t n16 = bytes / 16;
+    // one workgroup per WAVES chunks of the smallest chunk size (1 KB)
+    if ((n16 + WAVES * WAVE - 1) / (WAVES * WAVE) > 0x7FFFFFFFu) return INTSPMV_ERR_ARG;
+    int st = ispv_enter(ctx);
+    if (st) return st;
+    void *buf = nullptr;
+    double *out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t s = ctx->stream;
+    const size_t max_chunks = (n16 + WAVE - 1) / WAVE;
+    hipError_t e = hipMalloc(&buf, n16 * 16);
+    if (e == hipSuccess) e = hipMalloc(&out, (max_chunks + WAVES) * sizeof(double));
+    if (e == hipSuccess) e = hipMemsetAsync(buf, 0, n16 * 16, s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // the best of four chunk sizes per wave (1, 2, 4, 8 KB): which one the
+    // memory system serves fastest varies with the size read
+    double best = 0.0;
+    for (int U = 1; U <= 8 && e == hipSuccess; U *= 2) {
+        const size_t per_wg = (size_t)U * WAVE * WAVES;
+        const unsigned grid = (unsigned)((n16 + per_wg - 1) / per_wg);
+        auto launch = [&]() {
+            if (U == 1) calib_stream_kernel<1><<<grid, BLOCK_THREADS, 0, s>>>((const d2v *)buf, n16, out);
+            else if (U == 2) calib_stream_kernel<2><<<grid, BLOCK_THREADS, 0, s>>>((const d2v *)buf, n16, out);
+            else if (U == 4) calib_stream_kernel<4><<<grid, BLOCK_THREADS, 0, s>>>((const d2v *)buf, n16, out);
+            else calib_stream_kernel<8><<<grid, BLOCK_THREADS, 0, s>>>((const d2v *)buf, n16, out);
+            return hipGetLastError();
+        };
+        for (int w = 0; w < 2 && e == hipSuccess; w++) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (uint32_t r = 0; r < reps && e == hipSuccess; r++) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms > 0.0f) best = std::max(best, (double)(n16 * 16) * reps / (ms * 1e-3));
+    }
+    if (e == hipSuccess) *bytes_per_s = best;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    if (buf) (void)hipFree(buf);
+    if (out) (void)hipFree(out);
+    if (e != hipSuccess) {
+        ctx->last_err = e;
+        return e == hipErrorOutOfMemory ? INTSPMV_ERR_NOMEM : INTSPMV_ERR_HIP;
+    }
+    return INTSPMV_OK;
+}
 
 extern "C" int intspmv_gather_rate(intspmv_ctx *ctx, size_t span_bytes, uint32_t elem_bytes,
                                    size_t count, uint32_t reps, double *gathers_per_s)

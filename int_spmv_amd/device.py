@@ -97,6 +97,7 @@ SIGNATURES = [
     ("spmv_axt_h1_noalloc", C.c_int, [_vp, _u32, _u32, _vp, _vp, _vp]),
     ("intspmv_gather_rate", C.c_int, [_vp, C.c_size_t, _u32, C.c_size_t, _u32,
                                       C.POINTER(C.c_double)]),
+    ("intspmv_stream_rate", C.c_int, [_vp, C.c_size_t, _u32, C.POINTER(C.c_double)]),
     ("intspmv_plan_destroy", C.c_int, [_vp]),
     ("intspmv_plan_info", C.c_int, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
     ("intspmv_plan_layout", C.c_int, [_u32, _vp, _u32, _u32, _u32, _u32, _vp, C.c_size_t,
@@ -223,6 +224,13 @@ class Engine:
         r = C.c_double()
         self._check(lib().intspmv_gather_rate(self.h, span_bytes, elem_bytes, count, reps,
                                               C.byref(r)), "gather_rate")
+        return r.value
+
+    def stream_rate(self, nbytes: int, reps: int = 20) -> float:
+        """Bytes per second of a plain read of an nbytes buffer
+        (intspmv_stream_rate): the ceiling of the streaming products."""
+        r = C.c_double()
+        self._check(lib().intspmv_stream_rate(self.h, nbytes, reps, C.byref(r)), "stream_rate")
         return r.value
 
     def sync(self) -> None:

@@ -160,7 +160,7 @@ def test_bad_arguments_are_refused_before_launch(engine):
 def test_gather_rate_calibration():
     """intspmv_gather_rate (the gather ceiling bench.py reports) returns a
     positive rate for every element size and refuses bad arguments without
-    launching."""
+    launching; so does intspmv_stream_rate (the streaming ceiling)."""
     eng = D.Engine(0)
     for elem in (4, 8, 16):
         r = eng.gather_rate(1 << 20, elem, count=1 << 20, reps=2)
@@ -169,3 +169,9 @@ def test_gather_rate_calibration():
                                     (1 << 20, 8, 0, 1), (1 << 20, 8, 1 << 10, 0)):
         with pytest.raises(D.IntSpmvError):
             eng.gather_rate(span, elem, count=count, reps=reps)
+    # the streaming ceiling (intspmv_stream_rate): a plausible rate, bad sizes refused
+    r = eng.stream_rate(64 << 20, reps=3)
+    assert 1e11 < r < 2e13, r
+    for nbytes, reps in ((8, 1), (1 << 20, 0)):
+        with pytest.raises(D.IntSpmvError):
+            eng.stream_rate(nbytes, reps=reps)
