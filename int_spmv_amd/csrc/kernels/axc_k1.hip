@@ -64,6 +64,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
                 size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
                 v[j] = ntload(reinterpret_cast<const d2v *>(ax + T * 16 + 2 * p));
             }
+            loads_issued();
 #pragma unroll
             for (int j = 0; j < 2 * U; j++) {
                 double x0 = dpp_f64<DPP_ROW_SHL4>(v[j][0]), x1 = dpp_f64<DPP_ROW_SHL4>(v[j][1]);
@@ -80,15 +81,28 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axc_kernel(
         ra = brp[r0];
         rb = brp[r0 + 1];
     }
-    constexpr int UN = AXC_UMAX_DENSE / 8; // a dense-matrix block in flight at once
-    for (uint32_t base = 0; base < nu; base += 8 * UN) {
-        d2v v[UN];
+    // rounds of UN loads (a dense-matrix block); the rounds of a larger block
+    // (matrices with many empty rows) are all in flight together, and a round
+    // past the block's bricks (every round of a block of empty rows, whose
+    // clamp brick nu - 1 does not exist) loads nothing
+    constexpr int UN = AXC_UMAX_DENSE / 8, NR = AXC_UMAX / AXC_UMAX_DENSE;
+    d2v vr[NR][UN];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        if (nu <= 8 * UN * r) break; // uniform
 #pragma unroll
         for (int j = 0; j < UN; j++) {
-            uint32_t t = base + j * 8 + lane / 8;
+            uint32_t t = 8 * UN * r + j * 8 + lane / 8;
             size_t T = (size_t)d.ubeg + (t < nu ? t : nu - 1);
-            v[j] = ntload(reinterpret_cast<const d2v *>(ax + T * 16 + 2 * p));
+            vr[r][j] = ntload(reinterpret_cast<const d2v *>(ax + T * 16 + 2 * p));
         }
+    }
+    loads_issued();
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        if (nu <= 8 * UN * r) break; // uniform
+        const uint32_t base = 8 * UN * r;
+        d2v (&v)[UN] = vr[r];
 #pragma unroll
         for (int j = 0; j < UN; j++) {
             const bool ok = base + j * 8 + lane / 8 < nu && p < 4;
