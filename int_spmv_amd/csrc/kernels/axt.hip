@@ -916,17 +916,24 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_noc_fl32_kernel(
 #ifndef ISPV_PLANAR_STEPS_TH
 #define ISPV_PLANAR_STEPS_TH 2 // tile height > 1
 #endif
-// Four stored values (bf16 or FP32) of the value plane as floats.
+// Four stored values (bf16 or FP32) of the value plane: load4_raw reads
+// them as stored, as_floats widens them.  Apart, so that the widening of a
+// load's bf16 bits comes after all of a step's loads are issued.
 template <typename VT>
-__device__ __forceinline__ float4 load4_values(const VT *p)
+using Raw4 = typename std::conditional<sizeof(VT) == 2, u2v, float4>::type;
+template <typename VT>
+__device__ __forceinline__ Raw4<VT> load4_raw(const VT *p)
 {
-    if constexpr (sizeof(VT) == 2) {
-        const u2v b = ntload(reinterpret_cast<const u2v *>(p));
+    if constexpr (sizeof(VT) == 2) return ntload(reinterpret_cast<const u2v *>(p));
+    else return f4load(reinterpret_cast<const float *>(p));
+}
+template <typename VT>
+__device__ __forceinline__ float4 as_floats(const Raw4<VT> &b)
+{
+    if constexpr (sizeof(VT) == 2)
         return make_float4(__uint_as_float(b[0] << 16), __uint_as_float(b[0] & 0xFFFF0000u),
                            __uint_as_float(b[1] << 16), __uint_as_float(b[1] & 0xFFFF0000u));
-    } else {
-        return f4load(reinterpret_cast<const float *>(p));
-    }
+    else return b;
 }
 
 // Planar storage (value plane of bf16 or FP32, x plane of FP32), tile height
@@ -963,17 +970,20 @@ __global__ __launch_bounds__(BLOCK_THREADS) void axt_planar_kernel(
     constexpr int U = TH == 1 ? ISPV_PLANAR_STEPS
                     : (TH == 2 || TH == 4) ? (int)(UM * TH / 256) : ISPV_PLANAR_STEPS_TH;
     for (size_t hb = H0; hb < H1; hb += 64 * U) {
-        float4 v[U], x[U];
+        Raw4<VT> vb[U];
+        float4 x[U];
 #pragma unroll
         for (int j = 0; j < U; j++) {
             const size_t h = hb + 64 * j + lane, hc = h < H1 ? h : H1 - 1;
-            v[j] = load4_values<VT>(av + hc * 4);
+            vb[j] = load4_raw<VT>(av + hc * 4);
             x[j] = f4load(xp + hc * 4);
         }
+        loads_issued();
 #pragma unroll
         for (int j = 0; j < U; j++) {
             const size_t h = hb + 64 * j + lane;
-            float c[4] = {v[j].x * x[j].x, v[j].y * x[j].y, v[j].z * x[j].z, v[j].w * x[j].w};
+            const float4 v = as_floats<VT>(vb[j]);
+            float c[4] = {v.x * x[j].x, v.y * x[j].y, v.z * x[j].z, v.w * x[j].w};
             if constexpr (TH == 1) {
                 float t = (c[0] + c[1]) + (c[2] + c[3]);
                 t = t + dpp_f32<DPP_QUAD_XOR1>(t);
