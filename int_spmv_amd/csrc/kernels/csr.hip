@@ -65,17 +65,26 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
         double acc = 0.0;
         for (uint32_t c0 = 0; c0 < nu; c0 += CSR_UMAX) {
             const uint32_t m = nu - c0 < CSR_UMAX ? nu - c0 : CSR_UMAX;
+            // the chunk's loads, then its gathers, then its stores, each all
+            // issued together (branch-free: lanes past the chunk re-read its
+            // last entry and store nothing)
+            constexpr int NJ = CSR_UMAX / WAVE;
+            double v[NJ], xv[NJ];
+            uint32_t c[NJ];
 #pragma unroll
-            for (int j = 0; j < (int)(CSR_UMAX / WAVE); j++) {
-                uint32_t i = j * WAVE + lane;
-                const bool ok = i < m;
-                i = ok ? i : m - 1; // branch-free: re-read the chunk's last entry
-                const double v = ntload(val + d.ubeg + c0 + i);
-                const uint32_t c = ntload(col + d.ubeg + c0 + i);
-                const double xv = x[c];
-                if (ok) {
-                    s_v[w][i] = v;
-                    s_x[w][i] = xv;
+            for (int j = 0; j < NJ; j++) {
+                const uint32_t i = j * WAVE + lane < m ? j * WAVE + lane : m - 1;
+                v[j] = ntload(val + d.ubeg + c0 + i);
+                c[j] = ntload(col + d.ubeg + c0 + i);
+            }
+#pragma unroll
+            for (int j = 0; j < NJ; j++) xv[j] = x[c[j]];
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                const uint32_t i = j * WAVE + lane;
+                if (i < m) {
+                    s_v[w][i] = v[j];
+                    s_x[w][i] = xv[j];
                 }
             }
             wave_lds_sync();
@@ -100,9 +109,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) void csr_block_kernel(
                 v[j] = ntload(val + d.ubeg + i);
                 c[j] = ntload(col + d.ubeg + i);
             }
+            // all gathers issued before the chain (clamped lanes re-read a
+            // valid column), so they are in flight together
+            double xv[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) xv[j] = x[c[j]];
 #pragma unroll
             for (int j = 0; j < U; j++)
-                if (base + j * WAVE + lane < nu) acc = fma(v[j], x[c[j]], acc);
+                if (base + j * WAVE + lane < nu) acc = fma(v[j], xv[j], acc);
         }
         finish_long<double>(wave_sum_f64(acc), d, bid, y, nrows, partial);
         return;
