@@ -112,6 +112,60 @@ def test_random_matrix_all_products(engine, seed):
         assert scaled(m, X[:, j], O.cf_csr(m.row, m.col, m.val, X[:, j]), Yg[:, j]) <= 1e-12, (seed, j)
 
 
+@pytest.mark.parametrize("seed", range(24))
+def test_random_many_empty_rows(engine, seed):
+    """Matrices with 30-70 % empty rows take the plans of 128-unit blocks,
+    whose load rounds the th = 1 FP64 and AXC kernels keep in flight
+    together, and whose rows run up to 256 per block.  Rows in normal blocks
+    keep the reference kernels' bits (AXT NOC at th 1 and at a drawn th, AXC),
+    COM rows of at most 128 entries cf_CSR's; everything within 1e-12, FP32
+    storage within 1e-6 and bf16 values within 1e-2 of the FP64 oracle."""
+    from .test_gpu_fullsize import _many_empty, _normal_rows
+    rng = np.random.default_rng(5000 + seed)
+    n = int(rng.choice([500, 3000, 20000]))
+    lens = rng.integers(1, 60, size=n)
+    lens[rng.random(n) < rng.uniform(0.3, 0.7)] = 0
+    for r in rng.integers(0, n, size=3):  # long rows: some past 64 tiles
+        lens[r] = int(rng.integers(200, 3000))
+    row = np.zeros(n + 1, np.uint32)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.integers(0, n, size=k)) for k in lens]).astype(np.uint32)
+    m = H.CSR.from_arrays(row, col, rng.standard_normal(len(col)))
+    x = rng.uniform(-1, 1, n)
+    th = int(rng.choice([2, 4, 8, 16, 32, 48]))
+    bs = int(rng.choice([2, 128, 512]))
+    print(f"[random-empty] seed={seed} n={n} nnz={m.nnz} th={th} bs={bs}", flush=True)
+    assert _many_empty(m)
+    yr = O.cf_csr(m.row, m.col, m.val, x)
+    for t in (1, th):
+        a = H.AXT(m, x, th=t)
+        y = _y(engine, n)
+        D.DeviceAXT(engine, a).int_axt(y)
+        g = _host(y, n)
+        ok = _normal_rows(m, t)
+        assert bitwise_equal(g[ok], O.int_axt(a)[ok]), (seed, t)
+        assert scaled(m, x, yr, g) <= 1e-12, (seed, t)
+        if t == 1:
+            for vt, xt, tol in ((D.F32, D.F32, 1e-6), (D.BF16, D.F32, 1e-2)):
+                y = _y(engine, n, np.float32)
+                D.DeviceAXT(engine, a, val_type=vt, x_type=xt).int_axt(y)
+                assert scaled(m, x, yr, _host(y, n)) <= tol, (seed, vt)
+    c = H.AXC(m, x)
+    y = _y(engine, n)
+    D.DeviceAXC(engine, c).int_axc(y)
+    g = _host(y, n)
+    ok = np.diff(c.brp.astype(np.int64)) // 16 <= 64  # normal blocks of the sparse-row plan
+    assert bitwise_equal(g[ok], O.int_axc(n, c.ax, c.brp)[ok]), seed
+    assert scaled(m, x, yr, g) <= 1e-12, seed
+    a = H.AXT(m, x, th=1, mode="COM", bs=bs)
+    y = _y(engine, n)
+    D.DeviceAXT(engine, a).int_axt(y)
+    g = _host(y, n)
+    short = np.diff(m.row.astype(np.int64)) <= 128
+    assert bitwise_equal(g[short], yr[short]), seed
+    assert scaled(m, x, yr, g) <= 1e-12, seed
+
+
 @pytest.mark.parametrize("seed", range(18))
 def test_random_batch(engine, seed):
     """One batched launch over 1..12 of the random matrices, each with its own
