@@ -1,6 +1,6 @@
 """GPU: degenerate matrices through every entry point -- a 1x1 matrix, a
-matrix whose rows are all empty, a single empty row, and a matrix whose only
-entries sit in its last row.  y is pre-filled with NaN, so a row that no
+matrix whose rows are all empty, a single empty row, a matrix whose only
+entries sit in its last row, and runs of empty rows longer than a block.  y is pre-filled with NaN, so a row that no
 kernel writes fails; the expected values come from the oracle (cf_CSR), and
 with at most two products per row every format must give its bits."""
 import numpy as np
@@ -118,6 +118,66 @@ def test_every_format_on_degenerate_matrices(engine, name, m):
     for j in range(k3):
         want = O.cf_csr(m.row, m.col, v32, X[:, j].astype(np.float32).astype(np.float64))
         assert np.allclose(Ybh[:, j], want, rtol=1e-6, atol=0), (name, j)
+
+
+@pytest.mark.parametrize("lead", [64, 67])
+def test_long_runs_of_empty_rows(engine, lead):
+    """Runs of 600 and 300 empty rows (longer than a block's 256 rows, so
+    whole blocks hold no unit) after `lead` slots -- a multiple of 8 or not,
+    so an empty block starts on a tile boundary or inside a tile -- and at
+    the end.  Every format must write each empty row's 0 and keep its own
+    bits on the rows with entries (all short)."""
+    rng = np.random.default_rng(lead)
+    lens = np.concatenate([np.full(lead // 4, 4), [lead % 4] if lead % 4 else [],
+                           np.zeros(600), rng.integers(1, 20, 500), np.zeros(300)]).astype(np.int64)
+    n = len(lens)
+    row = np.zeros(n + 1, np.uint32)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.integers(0, n, size=k)) for k in lens]).astype(np.uint32)
+    m = H.CSR.from_arrays(row, col, rng.standard_normal(len(col)))
+    x = rng.uniform(-1, 1, n)
+    yr = O.cf_csr(m.row, m.col, m.val, x)
+    empty = lens == 0
+    xd = engine.upload(x)
+    got = {}
+    dc = D.DeviceCSR(engine, m)
+    for name, fn in (("csr_ordered", dc.cf_csr), ("csr", dc.spmv), ("csr_exact", dc.spmv_exact)):
+        y = _nan(engine, n)
+        fn(xd, y)
+        got[name] = (_host(y, n), yr)
+    c = H.AXC(m, x)
+    y = _nan(engine, n)
+    D.DeviceAXC(engine, c).int_axc(y)
+    got["axc"] = (_host(y, n), O.int_axc(n, c.ax, c.brp))
+    k = H.K1(m)
+    y = _nan(engine, n)
+    D.DeviceK1(engine, k).int_k1p(xd, y)
+    got["k1p"] = (_host(y, n), O.int_k1p(k.chunkNum, n, k.val, k.col, k.chp, k.nmc, k.permi, x))
+    for th in (1, 4, 16):
+        a = H.AXT(m, x, th=th)
+        y = _nan(engine, n)
+        D.DeviceAXT(engine, a).int_axt(y)
+        got[f"axt_th{th}"] = (_host(y, n), O.int_axt(a))
+        if th in (1, 4):
+            for vt, xt in ((D.F32, D.F32), (D.BF16, D.F32)):
+                y = _nan(engine, n, np.float32)
+                D.DeviceAXT(engine, a, val_type=vt, x_type=xt).int_axt(y)
+                g = _host(y, n)
+                assert np.all(g[empty] == 0.0), (th, vt)
+                assert np.all(np.isfinite(g)), (th, vt)
+    for bs in (2, 128):
+        a = H.AXT(m, x, th=1, mode="COM", bs=bs)
+        y = _nan(engine, n)
+        D.DeviceAXT(engine, a).int_axt(y)
+        got[f"com_bs{bs}"] = (_host(y, n), yr)
+    ag = H.AXT(m, np.zeros(n), th=1, want_acol=True)
+    g = D.DeviceAXTGather(engine, ag)
+    y = _nan(engine, n)
+    g.matvec(xd, y)
+    got["colmap"] = (_host(y, n), O.int_axt(H.AXT(m, x, th=1)))
+    for name, (g, ref) in got.items():
+        assert np.all(g[empty] == 0.0), name
+        assert bitwise_equal(g, ref), name
 
 
 def test_bad_arguments_are_refused_before_launch(engine):
