@@ -232,6 +232,6 @@ def test_gather_rate_calibration():
     # the streaming ceiling (intspmv_stream_rate): a plausible rate, bad sizes refused
     r = eng.stream_rate(64 << 20, reps=3)
     assert 1e11 < r < 2e13, r
-    for nbytes, reps in ((8, 1), (1 << 20, 0)):
+    for nbytes, reps in ((8, 1), (1 << 20, 0), (1 << 50, 1)):  # the last: too many workgroups
         with pytest.raises(D.IntSpmvError):
             eng.stream_rate(nbytes, reps=reps)
